@@ -1,0 +1,137 @@
+/* r3d_hip.h - C ABI of the MI355X (gfx950) Rast-3DMHD time-step library.
+ *
+ * One context holds the whole global domain of one run on one GPU.  The
+ * reference's npey x npez MPI decomposition is kept as a set of *virtual*
+ * ranks: upload and download move the reference's per-rank arrays (local
+ * extents nx = npx+2, ny = npy/npey+2, nz = npz/npez+2, x fastest, ghosts
+ * included), while the device keeps one padded global array per field.
+ *
+ * Every call returns 0 on success or a negative R3D_E* code; the message of
+ * the last failure on the calling thread is in r3d_hip_last_error().  No call
+ * exits the process.  A context is not thread-safe: callers serialise calls
+ * on it.  All work is queued on the context's own HIP stream.
+ *
+ * Reference interfaces replaced (reference include/gpu.hpp:35-49):
+ *   gpu_alloc(GpuSim&, const Derived&)        -> r3d_hip_create
+ *   gpu_upload(GpuSim&, const SimState&)       -> r3d_hip_set_metrics + r3d_hip_upload_rank
+ *   gpu_download(SimState&, GpuSim&)           -> r3d_hip_download_rank + r3d_hip_get_scalars
+ *   gpu_free(GpuSim&)                          -> r3d_hip_destroy
+ *   step_gpu(MPI_Comm, ..., SimState& scalars) -> r3d_hip_step (nstages = 3)
+ *   step_gpu(int nstages, MPI_Comm, ...)       -> r3d_hip_step (nstages = 1..3)
+ * The C++ mirror of those signatures is include/r3d_gpu.hpp.
+ */
+#ifndef R3D_HIP_H
+#define R3D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    R3D_OK = 0,
+    R3D_EINVAL = -1,   /* invalid argument or configuration */
+    R3D_EHIP = -2,     /* HIP runtime error (message has the HIP error string) */
+    R3D_ENOMEM = -3,   /* device allocation failed */
+    R3D_ESTATE = -4,   /* call not valid in the context's current state */
+    R3D_EUNSUP = -5    /* feature the reference rejects (src/fluxes.cpp:46-51) */
+};
+
+/* Field ids used by upload/download (gstate order of the primaries and the
+ * RK accumulators, reference include/fields.hpp:77-104). */
+enum {
+    R3D_RU = 0, R3D_RV, R3D_RW, R3D_RO, R3D_TT, R3D_BX, R3D_BY, R3D_BZ,
+    R3D_ZRU, R3D_ZRV, R3D_ZRW, R3D_ZRO, R3D_ZTT, R3D_ZBX, R3D_ZBY, R3D_ZBZ,
+    R3D_NFIELDS
+};
+
+/* Run configuration: the Params/Derived scalars the step reads
+ * (reference include/params.hpp:24-145, src/params.cpp:14-66). */
+typedef struct r3d_hip_cfg {
+    int npx, npy, npz;          /* global interior points */
+    int npey, npez;             /* virtual rank grid (npy%npey == 0, npz%npez == 0) */
+    int lmag, lrot;             /* physics switches */
+    int itcon, izcon, ibcon;    /* boundary selectors (reference src/bcon.cpp) */
+    int variant;                /* 0 = bit-exact (no FMA contraction), 1 = FMA-contracted */
+    int device;                 /* HIP device ordinal */
+    double c13, c23, c43;
+    double gam1, gam2, gam3, zeta1, zeta2;
+    double hx, h2x, hy, h2y, hz, h2z;
+    double re, repr, ore, cv, ocv, gamma, grav, theta, sff;
+    double rm, orm, obeta, omx, omz;
+    double tp, tc, hh, xp, yp;  /* plume boundary condition */
+    double dzf;                 /* f4(1/(npz-1)), the BCON flux spacing numerator */
+    double tb;                  /* bottom temperature from the static profile */
+} r3d_hip_cfg;
+
+/* Global 1-D metrics.  x arrays have npx+2 entries, y arrays npy+2, z arrays
+ * npz+2, indexed like the reference's arrays of a single rank covering the
+ * whole domain (interior 1..np, ghosts 0 and np+1).  Only interior entries
+ * are read by the time step. */
+typedef struct r3d_hip_metrics {
+    const double *exx, *dxxdx, *d2xxdx2, *ddx;
+    const double *wyy, *dyydy, *d2yydy2, *ddy;
+    const double *zee, *dzzdz, *d2zzdz2, *ddz, *rkapa, *dkapa;
+} r3d_hip_metrics;
+
+/* Run scalars (the SimState scalar mirror, reference src/main.cpp:242-249). */
+typedef struct r3d_hip_scalars {
+    double dt, timc, timt, timi, umach;
+    int nit;
+} r3d_hip_scalars;
+
+typedef struct r3d_hip_ctx r3d_hip_ctx;
+
+/* Allocate the device state for cfg on cfg->device. */
+int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out);
+int r3d_hip_destroy(r3d_hip_ctx* ctx);
+
+/* Upload the global metrics (copied; host arrays only borrowed). */
+int r3d_hip_set_metrics(r3d_hip_ctx* ctx, const r3d_hip_metrics* m);
+
+/* Upload one virtual rank's primaries (fields[R3D_RU..R3D_BZ]; B entries are
+ * ignored unless lmag), each nx*ny*nz doubles with ghosts.  Every rank must be
+ * uploaded before the first step; ghost cells on the global boundary (periodic
+ * images and the outer z planes) are taken from the ranks that own them. */
+int r3d_hip_upload_rank(r3d_hip_ctx* ctx, int rank, const double* const* fields);
+
+/* Upload the whole global state at once: fields[R3D_RU..R3D_BZ], each
+ * (npx+2)*(npy+2)*(npz+2) doubles, x fastest. */
+int r3d_hip_upload_global(r3d_hip_ctx* ctx, const double* const* fields);
+
+/* Download one virtual rank: fields[R3D_RU..R3D_ZBZ] (NULL entries skipped),
+ * each nx*ny*nz doubles.  Primaries come back with every ghost cell the
+ * reference's halo exchange would hold; RK accumulators hold the interior
+ * (ghost cells zero, as in a reference dump). */
+int r3d_hip_download_rank(r3d_hip_ctx* ctx, int rank, double* const* fields);
+int r3d_hip_download_global(r3d_hip_ctx* ctx, double* const* fields);
+
+int r3d_hip_set_scalars(r3d_hip_ctx* ctx, const r3d_hip_scalars* s);
+int r3d_hip_get_scalars(r3d_hip_ctx* ctx, r3d_hip_scalars* s);
+
+/* Advance nsteps RK3 steps (nstages = 3), or one step truncated after
+ * nstages = 1 or 2 stages (reference src/step.cpp:114-257).  Asynchronous on
+ * the context stream except when tc != 0 (the plume time factor is taken
+ * from timc on the host between steps). */
+int r3d_hip_step(r3d_hip_ctx* ctx, int nsteps, int nstages);
+
+/* Block until the context stream is idle. */
+int r3d_hip_sync(r3d_hip_ctx* ctx);
+
+/* Kernel timing: when enabled, each stage launch is bracketed by HIP events
+ * on the context stream; r3d_hip_kernel_stats returns, per kernel class
+ * (0 = fused stage kernel, 1 = boundary kernel, 2 = dt kernel, 3 = small
+ * kernels), the number of launches and their total milliseconds since the
+ * last reset. */
+int r3d_hip_set_timing(r3d_hip_ctx* ctx, int enable);
+int r3d_hip_kernel_stats(r3d_hip_ctx* ctx, int cls, long* launches, double* total_ms);
+
+/* Device bytes held by the context. */
+int r3d_hip_memory(r3d_hip_ctx* ctx, unsigned long long* bytes);
+
+/* Thread-local message of the last failed call. */
+const char* r3d_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,328 @@
+"""rast3dmhd_amd - thin Python binding of the gfx950 Rast-3DMHD time-step library.
+
+The product is the C ABI in include/r3d_hip.h (lib/librast3dmhd_hip.so) and
+the C++ host library (lib/librast3dmhd_host.so); this module only marshals
+arguments through ctypes.  It never falls back to a CPU path: if the HIP
+library is missing, load() raises.
+
+The package directory is named ``rast-3dmhd_amd`` (not an importable
+identifier); load it with ``rast3dmhd_amd = load_package()`` from
+__graft_entry__, or importlib with a file location.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "librast3dmhd_hip.so")
+HOST_LIB = os.path.join(LIB_DIR, "librast3dmhd_host.so")
+
+R3D_RU, R3D_RV, R3D_RW, R3D_RO, R3D_TT, R3D_BX, R3D_BY, R3D_BZ = range(8)
+FIELD_NAMES = ["ru", "rv", "rw", "ro", "tt", "bx", "by", "bz",
+               "zru", "zrv", "zrw", "zro", "ztt", "zbx", "zby", "zbz"]
+PRIMARIES = FIELD_NAMES[:8]
+
+
+class HipError(RuntimeError):
+    """A non-zero status from the C ABI; .code holds the R3D_E* value."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"r3d_hip error {code}: {msg}")
+        self.code = code
+
+
+class Cfg(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("npx", "npy", "npz", "npey", "npez", "lmag", "lrot", "itcon", "izcon",
+                 "ibcon", "variant", "device")] + \
+               [(n, ctypes.c_double) for n in
+                ("c13", "c23", "c43", "gam1", "gam2", "gam3", "zeta1", "zeta2",
+                 "hx", "h2x", "hy", "h2y", "hz", "h2z",
+                 "re", "repr", "ore", "cv", "ocv", "gamma", "grav", "theta", "sff",
+                 "rm", "orm", "obeta", "omx", "omz",
+                 "tp", "tc", "hh", "xp", "yp", "dzf", "tb")]
+
+
+METRIC_NAMES = ["exx", "dxxdx", "d2xxdx2", "ddx", "wyy", "dyydy", "d2yydy2", "ddy",
+                "zee", "dzzdz", "d2zzdz2", "ddz", "rkapa", "dkapa"]
+
+
+class Metrics(ctypes.Structure):
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_double)) for n in METRIC_NAMES]
+
+
+class Scalars(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("dt", "timc", "timt", "timi", "umach")] + \
+               [("nit", ctypes.c_int)]
+
+
+_hip = None
+
+
+def load():
+    """Load the HIP library (raises if it is not built)."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    if not os.path.exists(HIP_LIB):
+        raise RuntimeError(f"HIP library not built: {HIP_LIB} (run make -C rast-3dmhd_amd)")
+    L = ctypes.CDLL(HIP_LIB)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    L.r3d_hip_last_error.restype = ctypes.c_char_p
+    sigs = {
+        "r3d_hip_create": [ctypes.POINTER(Cfg), ctypes.POINTER(vp)],
+        "r3d_hip_destroy": [vp],
+        "r3d_hip_set_metrics": [vp, ctypes.POINTER(Metrics)],
+        "r3d_hip_upload_rank": [vp, ip, ctypes.POINTER(dp)],
+        "r3d_hip_upload_global": [vp, ctypes.POINTER(dp)],
+        "r3d_hip_download_rank": [vp, ip, ctypes.POINTER(dp)],
+        "r3d_hip_download_global": [vp, ctypes.POINTER(dp)],
+        "r3d_hip_set_scalars": [vp, ctypes.POINTER(Scalars)],
+        "r3d_hip_get_scalars": [vp, ctypes.POINTER(Scalars)],
+        "r3d_hip_step": [vp, ip, ip],
+        "r3d_hip_sync": [vp],
+        "r3d_hip_set_timing": [vp, ip],
+        "r3d_hip_kernel_stats": [vp, ip, ctypes.POINTER(ctypes.c_long),
+                                 ctypes.POINTER(ctypes.c_double)],
+        "r3d_hip_memory": [vp, ctypes.POINTER(ctypes.c_ulonglong)],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _hip = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise HipError(rc, load().r3d_hip_last_error().decode())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class HipSim:
+    """One device context (the whole global domain of a run on one GPU)."""
+
+    def __init__(self, cfg):
+        L = load()
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        _check(L.r3d_hip_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.npey, self.npez = cfg.npey, cfg.npez
+        self.nx = cfg.npx + 2
+        self.ny = cfg.npy // cfg.npey + 2
+        self.nz = cfg.npz // cfg.npez + 2
+        self.nf = 8 if cfg.lmag else 5
+
+    def close(self):
+        if self.h:
+            load().r3d_hip_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_metrics(self, m):
+        """m: dict of global 1-D metric arrays (METRIC_NAMES)."""
+        keep = {k: np.ascontiguousarray(m[k], dtype=np.float64) for k in METRIC_NAMES}
+        st = Metrics(*[_dptr(keep[k]) for k in METRIC_NAMES])
+        _check(load().r3d_hip_set_metrics(self.h, ctypes.byref(st)))
+
+    def upload_rank(self, rank, fields):
+        arrs = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in PRIMARIES[:self.nf]]
+        for a in arrs:
+            assert a.size == self.nx * self.ny * self.nz, "rank array has the wrong size"
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 8)(*[_dptr(a) for a in arrs] +
+                                                    [None] * (8 - len(arrs)))
+        _check(load().r3d_hip_upload_rank(self.h, rank, ptrs))
+
+    def upload_global(self, fields):
+        shape = (self.cfg.npz + 2, self.cfg.npy + 2, self.cfg.npx + 2)
+        arrs = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in PRIMARIES[:self.nf]]
+        for a in arrs:
+            assert a.size == np.prod(shape)
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 8)(*[_dptr(a) for a in arrs] +
+                                                    [None] * (8 - len(arrs)))
+        _check(load().r3d_hip_upload_global(self.h, ptrs))
+
+    def download_rank(self, rank, names=None):
+        names = names or FIELD_NAMES
+        out = {n: np.empty((self.nz, self.ny, self.nx)) for n in names}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 16)()
+        for n in names:
+            ptrs[FIELD_NAMES.index(n)] = _dptr(out[n])
+        _check(load().r3d_hip_download_rank(self.h, rank, ptrs))
+        return out
+
+    def download_global(self, names=None):
+        names = names or FIELD_NAMES
+        shape = (self.cfg.npz + 2, self.cfg.npy + 2, self.cfg.npx + 2)
+        out = {n: np.empty(shape) for n in names}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 16)()
+        for n in names:
+            ptrs[FIELD_NAMES.index(n)] = _dptr(out[n])
+        _check(load().r3d_hip_download_global(self.h, ptrs))
+        return out
+
+    def set_scalars(self, dt=0.0, timc=0.0, timt=0.0, timi=0.0, umach=0.0, nit=0):
+        s = Scalars(dt, timc, timt, timi, umach, int(nit))
+        _check(load().r3d_hip_set_scalars(self.h, ctypes.byref(s)))
+
+    def scalars(self):
+        s = Scalars()
+        _check(load().r3d_hip_get_scalars(self.h, ctypes.byref(s)))
+        return {"dt": s.dt, "timc": s.timc, "timt": s.timt, "timi": s.timi,
+                "umach": s.umach, "nit": s.nit}
+
+    def step(self, nsteps=1, nstages=3):
+        _check(load().r3d_hip_step(self.h, nsteps, nstages))
+
+    def sync(self):
+        _check(load().r3d_hip_sync(self.h))
+
+    def set_timing(self, on):
+        _check(load().r3d_hip_set_timing(self.h, 1 if on else 0))
+
+    def kernel_stats(self, cls):
+        n = ctypes.c_long()
+        ms = ctypes.c_double()
+        _check(load().r3d_hip_kernel_stats(self.h, cls, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def memory_bytes(self):
+        b = ctypes.c_ulonglong()
+        _check(load().r3d_hip_memory(self.h, ctypes.byref(b)))
+        return b.value
+
+
+# ----------------------------------------------------------------- host setup --
+_host = None
+
+
+def load_host():
+    """Load the host-setup library (raises if it is not built)."""
+    global _host
+    if _host is not None:
+        return _host
+    if not os.path.exists(HOST_LIB):
+        raise RuntimeError(f"host library not built: {HOST_LIB} (run make -C rast-3dmhd_amd)")
+    L = ctypes.CDLL(HOST_LIB)
+    vp = ctypes.c_void_p
+    L.r3d_host_create.restype = vp
+    L.r3d_host_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    L.r3d_host_free.argtypes = [vp]
+    L.r3d_host_last_error.restype = ctypes.c_char_p
+    L.r3d_host_cfg.argtypes = [vp, ctypes.POINTER(Cfg)]
+    L.r3d_host_metrics.argtypes = [vp, ctypes.POINTER(Metrics)]
+    L.r3d_host_state.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_double))]
+    L.r3d_host_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.r3d_host_tb.argtypes = [vp]
+    L.r3d_host_tb.restype = ctypes.c_double
+    L.r3d_host_rank_metrics.argtypes = [vp, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.POINTER(ctypes.c_double))]
+    L.r3d_host_write_gstate.argtypes = [vp, ctypes.c_int, ctypes.c_char_p,
+                                        ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
+                                        ctypes.POINTER(Scalars)]
+    _host = L
+    return L
+
+
+class Run:
+    """Host setup of one run from reference-style flags (include/r3d_host.h)."""
+
+    def __init__(self, args):
+        L = load_host()
+        argv = (ctypes.c_char_p * len(args))(*[a.encode() for a in args])
+        h = L.r3d_host_create(len(args), argv)
+        if not h:
+            raise HipError(-1, L.r3d_host_last_error().decode())
+        self.h = h
+        d = (ctypes.c_int * 8)()
+        L.r3d_host_dims(h, d)
+        (self.npx, self.npy, self.npz, self.npey, self.npez, self.lmag,
+         self.ntotal, self.nstep0) = list(d)
+        self.shape = (self.npz + 2, self.npy + 2, self.npx + 2)
+
+    def close(self):
+        if self.h:
+            load_host().r3d_host_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cfg(self):
+        c = Cfg()
+        if load_host().r3d_host_cfg(self.h, ctypes.byref(c)) != 0:
+            raise HipError(-1, load_host().r3d_host_last_error().decode())
+        return c
+
+    def metrics(self):
+        m = Metrics()
+        load_host().r3d_host_metrics(self.h, ctypes.byref(m))
+        n = {"x": self.npx + 2, "y": self.npy + 2, "z": self.npz + 2}
+        out = {}
+        for q, name in enumerate(METRIC_NAMES):
+            ln = n["x"] if q < 4 else n["y"] if q < 8 else n["z"]
+            out[name] = np.ctypeslib.as_array(getattr(m, name), shape=(ln,)).copy()
+        return out
+
+    def state(self):
+        """Global initial primaries, (npz+2, npy+2, npx+2) each (copies)."""
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 8)()
+        load_host().r3d_host_state(self.h, ptrs)
+        return {name: np.ctypeslib.as_array(ptrs[q], shape=self.shape).copy()
+                for q, name in enumerate(PRIMARIES)}
+
+    def tb(self):
+        return load_host().r3d_host_tb(self.h)
+
+    def rank_metrics(self, rank):
+        nry, nrz = self.npy // self.npey, self.npz // self.npez
+        lens = [self.npx + 2] * 4 + [nry + 2] * 4 + [nrz + 2] * 6
+        arrs = [np.zeros(n) for n in lens]
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 14)(*[_dptr(a) for a in arrs])
+        load_host().r3d_host_rank_metrics(self.h, rank, ptrs)
+        return dict(zip(METRIC_NAMES, arrs))
+
+    def write_gstate(self, rank, path, fields, scalars):
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 16)()
+        keep = []
+        for q, name in enumerate(FIELD_NAMES):
+            if name in fields:
+                a = np.ascontiguousarray(fields[name], dtype=np.float64)
+                keep.append(a)
+                ptrs[q] = _dptr(a)
+        s = Scalars(scalars["dt"], scalars["timc"], scalars["timt"], scalars["timi"],
+                    scalars["umach"], int(scalars["nit"]))
+        if load_host().r3d_host_write_gstate(self.h, rank, path.encode(), ptrs,
+                                             ctypes.byref(s)) != 0:
+            raise HipError(-1, load_host().r3d_host_last_error().decode())
+
+    def make_sim(self, variant=0):
+        """Create the device context and upload this run's metrics and state."""
+        cfg = self.cfg()
+        cfg.variant = variant
+        sim = HipSim(cfg)
+        sim.set_metrics(self.metrics())
+        sim.upload_global(self.state())
+        sim.set_scalars()
+        return sim

@@ -1,0 +1,164 @@
+// r3d_aux.hpp - boundary-plane, dt and bookkeeping kernels (gfx950).
+//
+// k_bound   : per stage, after k_stage.  BCON on the global top (k = 1) and
+//             bottom (k = npz) planes (reference src/bcon.cpp:12-141), the
+//             ghost images of the rewritten values, the deferred Z' mix on
+//             those planes, their dt limits in the last stage, and the
+//             1/RO flip of the two outer ghost planes that the reference's
+//             FLUXES performs on its full array (src/fluxes.cpp:133-138) but
+//             its halo exchange never refreshes.
+// k_dt      : dt limits of the current state when no fused value is
+//             available (first step, after an upload or a truncated step).
+// k_finish  : reduces the dt partials and advances the step counters.
+#pragma once
+
+#include "r3d_stage.hpp"
+
+namespace r3d {
+
+constexpr int BOUND_THREADS = 256;
+constexpr int DT_THREADS = 256;
+
+__global__ __launch_bounds__(BOUND_THREADS) void k_bound(const StageArgs a, const Consts C,
+                                                         const Metrics M, int partial_base) {
+    __shared__ double red[BOUND_THREADS / 64][NPART];
+    const Geo g = a.g;
+    const int side = blockIdx.y;  // 0 = top, 1 = bottom
+    const int W = g.npx + 2, H = g.npy + 2;
+    const int idx = blockIdx.x * BOUND_THREADS + threadIdx.x;
+    double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
+    const double dt = a.scal->dt;
+    const double cz = (a.stage < 2) ? C.zeta[a.stage] * dt : 0.0;
+
+    if (idx < W * H) {
+        const int i = idx % W, j = idx / W;
+        // outer ghost plane: RO <- 1/RO over the whole plane
+        const long no = g.at(i, j, side ? g.npz + 1 : 0);
+        a.xout.q[QRO][no] = 1.0 / a.xin.q[QRO][no];
+
+        if (i >= 1 && i <= g.npx && j >= 1 && j <= g.npy) {
+            const int kb = side ? g.npz : 1;
+            const long kof = g.pp * (long)kb;
+            const long n = g.at(i, j, kb);
+            const long d1 = side ? -g.pp : g.pp;
+            double* const* X = a.xout.q;
+            const double ro = X[QRO][n], ro1 = X[QRO][n + d1], ro2 = X[QRO][n + 2 * d1];
+            const double ru = ro * (C.c43 * X[QRU][n + d1] / ro1 - C.c13 * X[QRU][n + 2 * d1] / ro2);
+            const double rv = ro * (C.c43 * X[QRV][n + d1] / ro1 - C.c13 * X[QRV][n + 2 * d1] / ro2);
+            const double rw = 0.0;
+            double tt = X[QTT][n];
+            const double tt1 = X[QTT][n + d1], tt2 = X[QTT][n + 2 * d1];
+            if (side == 0) {
+                switch (C.top_mode) {
+                    case TOP_GAUSS_T:
+                        tt = 1.0 - (1.0 - C.tpr) * M.e1[i] / C.hh * M.e2[j] / C.hh;
+                        break;
+                    case TOP_GAUSS_FLUX:
+                        tt = C.c43 * tt1 - C.c13 * tt2 -
+                             C.c23 * M.dz_top *
+                                 (C.theta - (C.theta - C.tpr) * M.e1[i] / C.hh * M.e2[j] / C.hh);
+                        break;
+                    case TOP_CONST_T:
+                        tt = 1.0;
+                        if (C.half_t && i <= (g.npx + 2) / 2) tt = 1.0 - C.tp;
+                        break;
+                    case TOP_CONST_FLUX:
+                        tt = C.c43 * tt1 - C.c13 * tt2 - C.c23 * M.dz_top * C.theta;
+                        break;
+                    default:
+                        break;
+                }
+            } else {
+                if (C.izcon == 0)
+                    tt = C.tb;
+                else
+                    tt = C.c43 * tt1 - C.c13 * tt2 + C.c23 * M.dz_bot * C.theta / M.rkapa[kb];
+            }
+            store_with_images(X[QRU], g, i, j, kof, ru);
+            store_with_images(X[QRV], g, i, j, kof, rv);
+            store_with_images(X[QRW], g, i, j, kof, rw);
+            store_with_images(X[QTT], g, i, j, kof, tt);
+
+            if (a.zmode == 0 && a.stage < 2) {
+                // mix for the next stage with the post-BCON state
+                const long fo = (side ? 5 : 0) * g.pp + XOFF + i + g.px * j;
+                a.z.q[QRU][n] = ru + cz * a.fb[fo + 0 * g.pp];
+                a.z.q[QRV][n] = rv + cz * a.fb[fo + 1 * g.pp];
+                a.z.q[QRW][n] = rw + cz * a.fb[fo + 2 * g.pp];
+                a.z.q[QRO][n] = ro + cz * a.fb[fo + 3 * g.pp];
+                a.z.q[QTT][n] = tt + cz * a.fb[fo + 4 * g.pp];
+            }
+            if (a.stage == 2 && a.want_dt) {
+                const double dd = dmin(dmin(M.ddx[i], M.ddy[j]), M.ddz[kb]);
+                dt_terms(C, ru, rv, rw, ro, tt, dd, M.rkapa[kb], pm);
+            }
+        }
+    }
+    if (a.stage == 2 && a.want_dt)
+        block_partials<BOUND_THREADS / 64>(pm, red, a.partial,
+                                           partial_base + blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// dt limits of the state `s` (reference src/step.cpp:121-209), one partial
+// per block.
+__global__ __launch_bounds__(DT_THREADS) void k_dt(const Geo g, const State s, const Consts C,
+                                                   const Metrics M, double* partial) {
+    __shared__ double red[DT_THREADS / 64][NPART];
+    double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
+    const long total = (long)g.npx * g.npy * g.npz;
+    for (long q = (long)blockIdx.x * DT_THREADS + threadIdx.x; q < total;
+         q += (long)gridDim.x * DT_THREADS) {
+        const int i = 1 + (int)(q % g.npx);
+        const long rest = q / g.npx;
+        const int j = 1 + (int)(rest % g.npy);
+        const int k = 1 + (int)(rest / g.npy);
+        const long n = g.at(i, j, k);
+        const double dd = dmin(dmin(M.ddx[i], M.ddy[j]), M.ddz[k]);
+        dt_terms(C, s.q[QRU][n], s.q[QRV][n], s.q[QRW][n], s.q[QRO][n], s.q[QTT][n], dd,
+                 M.rkapa[k], pm);
+    }
+    block_partials<DT_THREADS / 64>(pm, red, partial, blockIdx.x);
+}
+
+// One block of 1024 threads.  reduce: fold partial[0..npart) into dt_next /
+// umach_next (dt = sff * min of the limits, src/step.cpp:224-229; umach =
+// sqrt(max), src/step.cpp:141).  advance: the end-of-step counters
+// (src/step.cpp:254-256).
+__global__ __launch_bounds__(1024) void k_finish(DevScal* sc, const double* partial, int npart,
+                                                 const Consts C, int reduce, int advance) {
+    __shared__ double red[16][NPART];
+    double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
+    if (reduce)
+        for (int q = threadIdx.x; q < npart; q += 1024) {
+            pm[0] = dmax(pm[0], partial[(long)q * NPART]);
+            for (int v = 1; v < NPART; ++v) pm[v] = dmin(pm[v], partial[(long)q * NPART + v]);
+        }
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    double v[NPART];
+    v[0] = wave_max(pm[0]);
+    for (int q = 1; q < NPART; ++q) v[q] = wave_min(pm[q]);
+    if (ln == 0)
+        for (int q = 0; q < NPART; ++q) red[wv][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (advance) {
+            sc->nit += 1;
+            sc->timc += sc->dt;
+            sc->timt = sc->timi + sc->timc;
+        }
+        if (reduce) {
+            double o[NPART];
+            for (int q = 0; q < NPART; ++q) o[q] = red[0][q];
+            for (int w = 1; w < 16; ++w) {
+                o[0] = dmax(o[0], red[w][0]);
+                for (int q = 1; q < NPART; ++q) o[q] = dmin(o[q], red[w][q]);
+            }
+            sc->umach_next = sqrt(o[0]);
+            double m = dmin(dmin(o[1], o[2]), o[3]);
+            if (C.lmag) m = dmin(m, o[4]);
+            sc->dt_next = C.sff * m;
+        }
+    }
+}
+
+}  // namespace r3d

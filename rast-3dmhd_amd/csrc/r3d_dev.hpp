@@ -1,0 +1,89 @@
+// r3d_dev.hpp - device-side data structures shared by the gfx950 kernels.
+//
+// Global layout of every 3-D field: one padded array per field covering the
+// whole domain with one ghost layer, x fastest.  Element (i, j, k) with
+// i in [0, npx+1], j in [0, npy+1], k in [0, npz+1] lives at
+//     XOFF + i + px*j + pp*k,   px = row pitch (multiple of 16 doubles),
+//                               pp = px*(npy+2)
+// XOFF = 15 puts the first interior column (i = 1) on a 128-byte boundary, so
+// a wavefront's 64 consecutive interior doubles are exactly four cache lines.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace r3d {
+
+constexpr int XOFF = 15;
+constexpr int NXF = 8;  // primaries per state: ru rv rw ro tt bx by bz
+
+// Indices of the primaries inside a state.
+enum { QRU = 0, QRV, QRW, QRO, QTT, QBX, QBY, QBZ };
+
+// Plume / boundary branch codes for the top plane (reference src/bcon.cpp:43-94).
+enum { TOP_GAUSS_T = 0, TOP_GAUSS_FLUX = 1, TOP_CONST_T = 2, TOP_CONST_FLUX = 3, TOP_NONE = 4 };
+
+struct Geo {
+    int npx, npy, npz;
+    long px, pp;
+    __host__ __device__ long at(int i, int j, int k) const {
+        return XOFF + i + px * j + pp * (long)k;
+    }
+};
+
+// Scalars of the run that live on the device, so that a step needs no host
+// round trip (dt of step n+1 is reduced at the end of step n).
+struct DevScal {
+    double dt, timc, timt, timi, umach;
+    double dt_next, umach_next;
+    int nit, pad;
+};
+
+// Constants of the step: the reference's Derived/Params values that the
+// kernels read, combined exactly as the reference combines them
+// (e.g. t1 = ocv*ore as in src/fluxes.cpp:344).
+struct Consts {
+    double c13, c23, c43, ore, ocv, grav;
+    double hx, h2x, hy, h2y, hz, h2z;
+    double omx, omz;
+    double tk;   // ocv/repr          (src/fluxes.cpp:247)
+    double t1;   // ocv*ore           (src/fluxes.cpp:344)
+    double t2;   // c13*ore           (src/fluxes.cpp:362)
+    double t3;   // c43*ocv*ore       (src/fluxes.cpp:399)
+    double og;   // 1/gamma           (src/step.cpp:119)
+    double gamma, re, repr, cv, sff, rm, obeta, orm;
+    double theta, tb, hh, tpr, tp;
+    double gam[3], zeta[2];
+    int lrot, lmag, top_mode, half_t, izcon, bz_zero;
+};
+
+// Metric arrays on the device (global indexing, see r3d_hip.h).
+struct Metrics {
+    const double *dxx, *d2xx, *ddx;
+    const double *dyy, *d2yy, *ddy;
+    const double *dzz, *d2zz, *ddz, *rkapa, *dkapa;
+    const double *e1, *e2;  // plume Gaussians exp(...) per x / per y (host libm)
+    double dz_top, dz_bot;  // f4(1/(npz-1)) / dzzdz at the boundary planes
+};
+
+struct State {
+    double* q[NXF];
+};
+
+// Arguments of one RK stage.
+struct StageArgs {
+    Geo g;
+    State xin, xout, z;
+    double* fb;        // F on the two BCON planes: [side][field][j][px]
+    double* partial;   // per-block dt partial extrema (stage 3)
+    DevScal* scal;
+    int stage;         // 0, 1, 2
+    int zmode;         // 0: Z' = mix (X' + zeta dt F); 1: Z' = X (save_state); 2: no Z write
+    int want_dt;       // stage 3 of a full step: emit dt partials of X'
+    int ntx, nty, nkc, kc, nblocks;
+};
+
+// Number of dt partial slots per block: max of (ogamma uu/tt), min of the
+// advective, thermal, viscous and (MHD) resistive limits.
+constexpr int NPART = 5;
+
+}  // namespace r3d

@@ -1,0 +1,620 @@
+// r3d_hip.hip - host side of the gfx950 time-step library: the C ABI of
+// include/r3d_hip.h over the kernels in r3d_stage.hpp / r3d_aux.hpp.
+//
+// Built with -ffp-contract=off (bit-exact variant).  Every HIP call is
+// checked; failures are returned as R3D_E* codes with a thread-local message,
+// never as process exits.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "r3d_hip.h"
+#include "r3d_aux.hpp"
+
+using namespace r3d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return set_err(R3D_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
+    } while (0)
+
+constexpr int TILE_Y = 8;  // rows per stage tile (waves 1..8 compute fluxes)
+constexpr int DT_BLOCKS = 1024;
+
+// Gather (dir = 1) or scatter (dir = 0) between the global array and one
+// virtual rank's local array (nx*ny*nz, x fastest).  Local (i, j, k) is global
+// (i, gj0 + j, gk0 + k).  Scatter writes only local j in [jlo, jhi] and k in
+// [klo, khi]; gather with interior_only leaves zeros outside the interior.
+__global__ void k_rank_copy(double* glob, double* loc, Geo g, int nx, int ny, int nz, int gj0,
+                            int gk0, int dir, int jlo, int jhi, int klo, int khi,
+                            int interior_only) {
+    const long total = (long)nx * ny * nz;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % nx);
+        const long rest = q / nx;
+        const int j = (int)(rest % ny);
+        const int k = (int)(rest / ny);
+        const long n = g.at(i, gj0 + j, gk0 + k);
+        if (dir == 0) {
+            if (j >= jlo && j <= jhi && k >= klo && k <= khi) glob[n] = loc[q];
+        } else {
+            const bool in = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2;
+            loc[q] = (interior_only && !in) ? 0.0 : glob[n];
+        }
+    }
+}
+
+}  // namespace
+
+struct r3d_hip_ctx {
+    r3d_hip_cfg cfg;
+    Geo g;
+    int nf = 5;
+    int nx = 0, ny = 0, nz = 0, nry = 0, nrz = 0;
+    hipStream_t stream = nullptr;
+    double* xs[2][NXF] = {};
+    double* z[NXF] = {};
+    int cur = 0;
+    double* fb = nullptr;
+    double* partial = nullptr;
+    DevScal* scal = nullptr;
+    double* metric_buf = nullptr;
+    double* stage_buf = nullptr;
+    Metrics M{};
+    Consts C{};
+    bool metrics_set = false;
+    bool next_valid = false;
+    std::vector<char> uploaded;
+    int ntx = 0, nty = 0, nkc = 0, kc = 0, nblocks = 0, grid = 0, nbb = 0;
+    unsigned long long bytes = 0;
+    bool timing = false;
+    long launches[4] = {0, 0, 0, 0};
+    double total_ms[4] = {0, 0, 0, 0};
+    std::vector<std::pair<int, hipEvent_t>> pending;  // (class, start event), end follows
+    std::vector<hipEvent_t> pending_end;
+};
+
+namespace {
+
+int dev_alloc(r3d_hip_ctx* c, double** p, size_t n) {
+    hipError_t e = hipMalloc((void**)p, n * sizeof(double));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(R3D_ENOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(double),
+                       hipGetErrorString(e));
+    }
+    c->bytes += n * sizeof(double);
+    return R3D_OK;
+}
+
+size_t field_elems(const Geo& g) { return (size_t)g.pp * (g.npz + 2) + 16; }
+
+void fill_consts(r3d_hip_ctx* c) {
+    const r3d_hip_cfg& f = c->cfg;
+    Consts& C = c->C;
+    C.c13 = f.c13;
+    C.c23 = f.c23;
+    C.c43 = f.c43;
+    C.ore = f.ore;
+    C.ocv = f.ocv;
+    C.grav = f.grav;
+    C.hx = f.hx;
+    C.h2x = f.h2x;
+    C.hy = f.hy;
+    C.h2y = f.h2y;
+    C.hz = f.hz;
+    C.h2z = f.h2z;
+    C.omx = f.omx;
+    C.omz = f.omz;
+    C.tk = f.ocv / f.repr;
+    C.t1 = f.ocv * f.ore;
+    C.t2 = f.c13 * f.ore;
+    C.t3 = f.c43 * f.ocv * f.ore;
+    C.og = 1.0 / f.gamma;
+    C.gamma = f.gamma;
+    C.re = f.re;
+    C.repr = f.repr;
+    C.cv = f.cv;
+    C.sff = f.sff;
+    C.rm = f.rm;
+    C.obeta = f.obeta;
+    C.orm = f.orm;
+    C.theta = f.theta;
+    C.tb = f.tb;
+    C.hh = f.hh;
+    C.tp = f.tp;
+    C.tpr = f.tp;
+    C.gam[0] = f.gam1;
+    C.gam[1] = f.gam2;
+    C.gam[2] = f.gam3;
+    C.zeta[0] = f.zeta1;
+    C.zeta[1] = f.zeta2;
+    C.lrot = f.lrot;
+    C.lmag = f.lmag;
+    C.izcon = f.izcon;
+    C.bz_zero = f.lmag && f.ibcon == 0;
+    // top boundary branch (reference src/bcon.cpp:43-94)
+    C.half_t = 0;
+    if (f.tp != 0.0 && (f.itcon == 0 || f.itcon == 1)) {
+        C.top_mode = f.itcon == 0 ? TOP_GAUSS_T : TOP_GAUSS_FLUX;
+    } else if (f.itcon == 0 || f.itcon == 2 || f.itcon == 4) {
+        C.top_mode = TOP_CONST_T;
+        C.half_t = f.itcon == 4;
+    } else if (f.itcon == 1) {
+        C.top_mode = TOP_CONST_FLUX;
+    } else {
+        C.top_mode = TOP_NONE;
+    }
+}
+
+// Time the launch made by fn() when timing is on.
+template <class Fn>
+int timed(r3d_hip_ctx* c, int cls, Fn fn) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, c->stream));
+    }
+    fn();
+    HIPCHK(hipGetLastError());
+    if (c->timing) {
+        HIPCHK(hipEventRecord(e1, c->stream));
+        c->pending.push_back({cls, e0});
+        c->pending_end.push_back(e1);
+    }
+    return R3D_OK;
+}
+
+int drain_timing(r3d_hip_ctx* c) {
+    for (size_t q = 0; q < c->pending.size(); ++q) {
+        HIPCHK(hipEventSynchronize(c->pending_end[q]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->pending[q].second, c->pending_end[q]));
+        c->launches[c->pending[q].first] += 1;
+        c->total_ms[c->pending[q].first] += ms;
+        (void)hipEventDestroy(c->pending[q].second);
+        (void)hipEventDestroy(c->pending_end[q]);
+    }
+    c->pending.clear();
+    c->pending_end.clear();
+    return R3D_OK;
+}
+
+StageArgs stage_args(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
+    StageArgs a{};
+    a.g = c->g;
+    for (int f = 0; f < NXF; ++f) {
+        a.xin.q[f] = c->xs[c->cur][f];
+        a.xout.q[f] = c->xs[c->cur ^ 1][f];
+        a.z.q[f] = c->z[f];
+    }
+    a.fb = c->fb;
+    a.partial = c->partial;
+    a.scal = c->scal;
+    a.stage = stage;
+    a.zmode = zmode;
+    a.want_dt = want_dt;
+    a.ntx = c->ntx;
+    a.nty = c->nty;
+    a.nkc = c->nkc;
+    a.kc = c->kc;
+    a.nblocks = c->nblocks;
+    return a;
+}
+
+int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
+    const StageArgs a = stage_args(c, stage, zmode, want_dt);
+    const dim3 blk((TILE_Y + 3) * 64);
+    int rc = timed(c, 0, [&] {
+        if (stage == 0)
+            hipLaunchKernelGGL((k_stage<TILE_Y, 0>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M);
+        else if (stage == 1)
+            hipLaunchKernelGGL((k_stage<TILE_Y, 1>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M);
+        else
+            hipLaunchKernelGGL((k_stage<TILE_Y, 2>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M);
+    });
+    if (rc) return rc;
+    rc = timed(c, 1, [&] {
+        hipLaunchKernelGGL(k_bound, dim3(c->nbb, 2), dim3(BOUND_THREADS), 0, c->stream, a, c->C,
+                           c->M, c->nblocks);
+    });
+    if (rc) return rc;
+    c->cur ^= 1;
+    return R3D_OK;
+}
+
+int launch_dt(r3d_hip_ctx* c) {
+    State s;
+    for (int f = 0; f < NXF; ++f) s.q[f] = c->xs[c->cur][f];
+    int rc = timed(c, 2, [&] {
+        hipLaunchKernelGGL(k_dt, dim3(DT_BLOCKS), dim3(DT_THREADS), 0, c->stream, c->g, s, c->C,
+                           c->M, c->partial);
+    });
+    if (rc) return rc;
+    return timed(c, 3, [&] {
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, c->stream, c->scal, c->partial,
+                           DT_BLOCKS, c->C, 1, 0);
+    });
+}
+
+int check_ready(r3d_hip_ctx* c) {
+    if (!c->metrics_set) return set_err(R3D_ESTATE, "metrics not uploaded");
+    for (size_t r = 0; r < c->uploaded.size(); ++r)
+        if (!c->uploaded[r]) return set_err(R3D_ESTATE, "virtual rank %zu not uploaded", r);
+    return R3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* r3d_hip_last_error(void) { return g_err.c_str(); }
+
+int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
+    if (!cfg || !out) return set_err(R3D_EINVAL, "null argument");
+    *out = nullptr;
+    const r3d_hip_cfg& f = *cfg;
+    if (f.npx < 3 || f.npy < 3 || f.npz < 6)
+        return set_err(R3D_EINVAL, "grid %dx%dx%d too small (need npx,npy >= 3, npz >= 6)", f.npx,
+                       f.npy, f.npz);
+    if (f.npey < 1 || f.npez < 1 || f.npy % f.npey || f.npz % f.npez)
+        return set_err(R3D_EINVAL, "npey=%d / npez=%d must divide npy=%d / npz=%d", f.npey, f.npez,
+                       f.npy, f.npz);
+    if (f.npz / f.npez < 3) return set_err(R3D_EINVAL, "each virtual z rank needs >= 3 planes");
+    if (f.lmag) return set_err(R3D_EUNSUP, "FLUXES: unsupported feature enabled: LMAG");
+    if (f.izcon != 0 && f.izcon != 1) return set_err(R3D_EINVAL, "BCON: invalid IZCON %d", f.izcon);
+    if (f.itcon == 2 || f.itcon == 3)
+        return set_err(R3D_EUNSUP, "FLUXES: unsupported feature enabled: ITCON embedded heat/thermal");
+    if (f.variant != 0) return set_err(R3D_EUNSUP, "only the bit-exact variant is built");
+    HIPCHK(hipSetDevice(f.device));
+
+    auto* c = new r3d_hip_ctx();
+    c->cfg = f;
+    c->nf = f.lmag ? 8 : 5;
+    c->g.npx = f.npx;
+    c->g.npy = f.npy;
+    c->g.npz = f.npz;
+    c->g.px = ((long)f.npx + 17 + 15) / 16 * 16;
+    c->g.pp = c->g.px * (f.npy + 2);
+    c->nry = f.npy / f.npey;
+    c->nrz = f.npz / f.npez;
+    c->nx = f.npx + 2;
+    c->ny = c->nry + 2;
+    c->nz = c->nrz + 2;
+    c->uploaded.assign((size_t)f.npey * f.npez, 0);
+    fill_consts(c);
+
+    // stage tiling: 64 x TILE_Y columns, z chunks sized for >= ~8 blocks per CU
+    c->ntx = (f.npx + 63) / 64;
+    c->nty = (f.npy + TILE_Y - 1) / TILE_Y;
+    c->kc = 8;
+    for (int kc : {128, 64, 32, 16, 8}) {
+        if ((long)c->ntx * c->nty * ((f.npz + kc - 1) / kc) >= 2048) {
+            c->kc = kc;
+            break;
+        }
+    }
+    c->nkc = (f.npz + c->kc - 1) / c->kc;
+    c->nblocks = c->ntx * c->nty * c->nkc;
+    c->grid = (c->nblocks + 7) / 8 * 8;
+    c->nbb = ((f.npx + 2) * (f.npy + 2) + BOUND_THREADS - 1) / BOUND_THREADS;
+
+    int rc = R3D_OK;
+    auto fail = [&](int code) {
+        r3d_hip_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(R3D_EHIP, "hipStreamCreate failed"));
+    const size_t fe = field_elems(c->g);
+    for (int f2 = 0; f2 < c->nf; ++f2) {
+        if ((rc = dev_alloc(c, &c->xs[0][f2], fe))) return fail(rc);
+        if ((rc = dev_alloc(c, &c->xs[1][f2], fe))) return fail(rc);
+        if ((rc = dev_alloc(c, &c->z[f2], fe))) return fail(rc);
+        if (hipMemsetAsync(c->z[f2], 0, fe * sizeof(double), c->stream) != hipSuccess)
+            return fail(set_err(R3D_EHIP, "hipMemset failed"));
+    }
+    if ((rc = dev_alloc(c, &c->fb, (size_t)2 * c->nf * c->g.pp + 16))) return fail(rc);
+    const size_t npart = std::max<size_t>((size_t)c->nblocks + 2 * c->nbb, DT_BLOCKS);
+    if ((rc = dev_alloc(c, &c->partial, npart * NPART))) return fail(rc);
+    {
+        void* p = nullptr;
+        if (hipMalloc(&p, sizeof(DevScal)) != hipSuccess)
+            return fail(set_err(R3D_ENOMEM, "hipMalloc scalars failed"));
+        c->scal = (DevScal*)p;
+        c->bytes += sizeof(DevScal);
+        DevScal z{};
+        if (hipMemcpy(c->scal, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(R3D_EHIP, "hipMemcpy scalars failed"));
+    }
+    const size_t nmet = 3 * (size_t)(f.npx + 2) + 3 * (size_t)(f.npy + 2) + 5 * (size_t)(f.npz + 2) +
+                        (size_t)(f.npx + 2) + (size_t)(f.npy + 2);
+    if ((rc = dev_alloc(c, &c->metric_buf, nmet))) return fail(rc);
+    const size_t stage_elems = (size_t)std::max<long>((long)c->nx * c->ny * c->nz,
+                                                      (long)(f.npx + 2) * (f.npy + 2));
+    if ((rc = dev_alloc(c, &c->stage_buf, stage_elems))) return fail(rc);
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(set_err(R3D_EHIP, "stream sync failed"));
+    *out = c;
+    return R3D_OK;
+}
+
+int r3d_hip_destroy(r3d_hip_ctx* c) {
+    if (!c) return R3D_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) (void)hipEventDestroy(p.second);
+    for (auto& e : c->pending_end) (void)hipEventDestroy(e);
+    for (int s = 0; s < 2; ++s)
+        for (int f = 0; f < NXF; ++f)
+            if (c->xs[s][f]) (void)hipFree(c->xs[s][f]);
+    for (int f = 0; f < NXF; ++f)
+        if (c->z[f]) (void)hipFree(c->z[f]);
+    if (c->fb) (void)hipFree(c->fb);
+    if (c->partial) (void)hipFree(c->partial);
+    if (c->scal) (void)hipFree(c->scal);
+    if (c->metric_buf) (void)hipFree(c->metric_buf);
+    if (c->stage_buf) (void)hipFree(c->stage_buf);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return R3D_OK;
+}
+
+int r3d_hip_set_metrics(r3d_hip_ctx* c, const r3d_hip_metrics* m) {
+    if (!c || !m) return set_err(R3D_EINVAL, "null argument");
+    const r3d_hip_cfg& f = c->cfg;
+    const int X = f.npx + 2, Y = f.npy + 2, Z = f.npz + 2;
+    const double* need[] = {m->exx, m->dxxdx, m->d2xxdx2, m->ddx, m->wyy, m->dyydy, m->d2yydy2,
+                            m->ddy, m->dzzdz, m->d2zzdz2, m->ddz, m->rkapa, m->dkapa};
+    for (const double* p : need)
+        if (!p) return set_err(R3D_EINVAL, "metric array missing");
+    // plume Gaussians, evaluated with the host libm exactly as the reference
+    // does per boundary cell (src/bcon.cpp:52-55 for ITCON=0, :67-68 otherwise)
+    const double cln = (double)(-4.0f * logf(2.0f)) / (f.hh * f.hh);
+    std::vector<double> e1(X), e2(Y);
+    for (int i = 0; i < X; ++i) {
+        const double dx = m->exx[i] - f.xp;
+        e1[i] = c->C.top_mode == TOP_GAUSS_T ? exp(cln * dx * dx) : exp(cln * (dx * dx));
+    }
+    for (int j = 0; j < Y; ++j) {
+        const double dy = m->wyy[j] - f.yp;
+        e2[j] = c->C.top_mode == TOP_GAUSS_T ? exp(cln * dy * dy) : exp(cln * (dy * dy));
+    }
+    std::vector<double> h;
+    h.reserve(3 * X + 3 * Y + 5 * Z + X + Y);
+    auto put = [&](const double* p, int n) -> const double* {
+        const size_t off = h.size();
+        h.insert(h.end(), p, p + n);
+        return c->metric_buf + off;
+    };
+    Metrics& M = c->M;
+    M.dxx = put(m->dxxdx, X);
+    M.d2xx = put(m->d2xxdx2, X);
+    M.ddx = put(m->ddx, X);
+    M.dyy = put(m->dyydy, Y);
+    M.d2yy = put(m->d2yydy2, Y);
+    M.ddy = put(m->ddy, Y);
+    M.dzz = put(m->dzzdz, Z);
+    M.d2zz = put(m->d2zzdz2, Z);
+    M.ddz = put(m->ddz, Z);
+    M.rkapa = put(m->rkapa, Z);
+    M.dkapa = put(m->dkapa, Z);
+    M.e1 = put(e1.data(), X);
+    M.e2 = put(e2.data(), Y);
+    M.dz_top = f.dzf / m->dzzdz[1];
+    M.dz_bot = f.dzf / m->dzzdz[f.npz];
+    HIPCHK(hipMemcpyAsync(c->metric_buf, h.data(), h.size() * sizeof(double),
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->metrics_set = true;
+    c->next_valid = false;
+    return R3D_OK;
+}
+
+int r3d_hip_upload_rank(r3d_hip_ctx* c, int rank, const double* const* fields) {
+    if (!c || !fields) return set_err(R3D_EINVAL, "null argument");
+    const r3d_hip_cfg& f = c->cfg;
+    if (rank < 0 || rank >= f.npey * f.npez) return set_err(R3D_EINVAL, "rank %d out of range", rank);
+    const int py = rank % f.npey, pz = rank / f.npey;
+    const int jlo = py == 0 ? 0 : 1, jhi = py == f.npey - 1 ? c->ny - 1 : c->ny - 2;
+    const int klo = pz == 0 ? 0 : 1, khi = pz == f.npez - 1 ? c->nz - 1 : c->nz - 2;
+    const size_t n = (size_t)c->nx * c->ny * c->nz;
+    for (int q = 0; q < c->nf; ++q) {
+        if (!fields[q]) return set_err(R3D_EINVAL, "field %d missing", q);
+        HIPCHK(hipMemcpyAsync(c->stage_buf, fields[q], n * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        for (int s = 0; s < 2; ++s)
+            hipLaunchKernelGGL(k_rank_copy, dim3(1024), dim3(256), 0, c->stream, c->xs[s][q],
+                               c->stage_buf, c->g, c->nx, c->ny, c->nz, py * c->nry, pz * c->nrz,
+                               0, jlo, jhi, klo, khi, 0);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->uploaded[rank] = 1;
+    c->next_valid = false;
+    return R3D_OK;
+}
+
+int r3d_hip_upload_global(r3d_hip_ctx* c, const double* const* fields) {
+    if (!c || !fields) return set_err(R3D_EINVAL, "null argument");
+    const r3d_hip_cfg& f = c->cfg;
+    const size_t W = (size_t)(f.npx + 2), rows = (size_t)(f.npy + 2) * (f.npz + 2);
+    for (int q = 0; q < c->nf; ++q) {
+        if (!fields[q]) return set_err(R3D_EINVAL, "field %d missing", q);
+        for (int s = 0; s < 2; ++s)
+            HIPCHK(hipMemcpy2DAsync(c->xs[s][q] + XOFF, c->g.px * sizeof(double), fields[q],
+                                    W * sizeof(double), W * sizeof(double), rows,
+                                    hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::fill(c->uploaded.begin(), c->uploaded.end(), 1);
+    c->next_valid = false;
+    return R3D_OK;
+}
+
+int r3d_hip_download_rank(r3d_hip_ctx* c, int rank, double* const* fields) {
+    if (!c || !fields) return set_err(R3D_EINVAL, "null argument");
+    const r3d_hip_cfg& f = c->cfg;
+    if (rank < 0 || rank >= f.npey * f.npez) return set_err(R3D_EINVAL, "rank %d out of range", rank);
+    const int py = rank % f.npey, pz = rank / f.npey;
+    const size_t n = (size_t)c->nx * c->ny * c->nz;
+    for (int q = 0; q < R3D_NFIELDS; ++q) {
+        if (!fields[q]) continue;
+        const bool isz = q >= R3D_ZRU;
+        const int fi = isz ? q - R3D_ZRU : q;
+        if (fi >= c->nf) {
+            std::memset(fields[q], 0, n * sizeof(double));
+            continue;
+        }
+        double* src = isz ? c->z[fi] : c->xs[c->cur][fi];
+        hipLaunchKernelGGL(k_rank_copy, dim3(1024), dim3(256), 0, c->stream, src, c->stage_buf,
+                           c->g, c->nx, c->ny, c->nz, py * c->nry, pz * c->nrz, 1, 0, 0, 0, 0,
+                           isz ? 1 : 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(fields[q], c->stage_buf, n * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return R3D_OK;
+}
+
+int r3d_hip_download_global(r3d_hip_ctx* c, double* const* fields) {
+    if (!c || !fields) return set_err(R3D_EINVAL, "null argument");
+    const r3d_hip_cfg& f = c->cfg;
+    const size_t W = (size_t)(f.npx + 2), rows = (size_t)(f.npy + 2) * (f.npz + 2);
+    for (int q = 0; q < R3D_NFIELDS; ++q) {
+        if (!fields[q]) continue;
+        const bool isz = q >= R3D_ZRU;
+        const int fi = isz ? q - R3D_ZRU : q;
+        if (fi >= c->nf) {
+            std::memset(fields[q], 0, W * rows * sizeof(double));
+            continue;
+        }
+        double* src = isz ? c->z[fi] : c->xs[c->cur][fi];
+        HIPCHK(hipMemcpy2DAsync(fields[q], W * sizeof(double), src + XOFF,
+                                c->g.px * sizeof(double), W * sizeof(double), rows,
+                                hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return R3D_OK;
+}
+
+int r3d_hip_set_scalars(r3d_hip_ctx* c, const r3d_hip_scalars* s) {
+    if (!c || !s) return set_err(R3D_EINVAL, "null argument");
+    DevScal d{};
+    HIPCHK(hipMemcpyAsync(&d, c->scal, sizeof d, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    d.dt = s->dt;
+    d.timc = s->timc;
+    d.timt = s->timt;
+    d.timi = s->timi;
+    d.umach = s->umach;
+    d.nit = s->nit;
+    HIPCHK(hipMemcpyAsync(c->scal, &d, sizeof d, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return R3D_OK;
+}
+
+int r3d_hip_get_scalars(r3d_hip_ctx* c, r3d_hip_scalars* s) {
+    if (!c || !s) return set_err(R3D_EINVAL, "null argument");
+    DevScal d{};
+    HIPCHK(hipMemcpyAsync(&d, c->scal, sizeof d, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    s->dt = d.dt;
+    s->timc = d.timc;
+    s->timt = d.timt;
+    s->timi = d.timi;
+    s->umach = d.umach;
+    s->nit = d.nit;
+    return R3D_OK;
+}
+
+int r3d_hip_step(r3d_hip_ctx* c, int nsteps, int nstages) {
+    if (!c) return set_err(R3D_EINVAL, "null context");
+    if (nstages < 1 || nstages > 3) return set_err(R3D_EINVAL, "nstages must be 1..3");
+    if (nsteps < 0 || (nstages < 3 && nsteps > 1))
+        return set_err(R3D_EINVAL, "a truncated step (nstages < 3) advances exactly one step");
+    int rc = check_ready(c);
+    if (rc) return rc;
+    for (int st = 0; st < nsteps; ++st) {
+        if (c->cfg.tc != 0.0) {
+            // time-dependent plume temperature (src/bcon.cpp:45) from the host libm
+            DevScal d{};
+            HIPCHK(hipMemcpyAsync(&d, c->scal, sizeof d, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            const double timt = d.timc + d.timi;
+            c->C.tpr = c->cfg.tp + (1.0 - c->cfg.tp) * exp(-timt / c->cfg.tc);
+        }
+        if (!c->next_valid && (rc = launch_dt(c))) return rc;
+        for (int s = 0; s < nstages; ++s) {
+            const int last = s == nstages - 1;
+            const int zmode = !last ? 0 : (s == 0 ? 1 : 2);
+            if ((rc = launch_stage(c, s, zmode, nstages == 3 && s == 2))) return rc;
+        }
+        rc = timed(c, 3, [&] {
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, c->stream, c->scal, c->partial,
+                               c->nblocks + 2 * c->nbb, c->C, nstages == 3 ? 1 : 0, 1);
+        });
+        if (rc) return rc;
+        c->next_valid = nstages == 3;
+    }
+    if (c->timing) return drain_timing(c);
+    return R3D_OK;
+}
+
+int r3d_hip_sync(r3d_hip_ctx* c) {
+    if (!c) return set_err(R3D_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return c->timing ? drain_timing(c) : R3D_OK;
+}
+
+int r3d_hip_set_timing(r3d_hip_ctx* c, int enable) {
+    if (!c) return set_err(R3D_EINVAL, "null context");
+    int rc = drain_timing(c);
+    if (rc) return rc;
+    c->timing = enable != 0;
+    for (int q = 0; q < 4; ++q) {
+        c->launches[q] = 0;
+        c->total_ms[q] = 0.0;
+    }
+    return R3D_OK;
+}
+
+int r3d_hip_kernel_stats(r3d_hip_ctx* c, int cls, long* launches, double* total_ms) {
+    if (!c || cls < 0 || cls > 3) return set_err(R3D_EINVAL, "bad argument");
+    int rc = drain_timing(c);
+    if (rc) return rc;
+    if (launches) *launches = c->launches[cls];
+    if (total_ms) *total_ms = c->total_ms[cls];
+    return R3D_OK;
+}
+
+int r3d_hip_memory(r3d_hip_ctx* c, unsigned long long* bytes) {
+    if (!c || !bytes) return set_err(R3D_EINVAL, "null argument");
+    *bytes = c->bytes;
+    return R3D_OK;
+}
+
+}  // extern "C"

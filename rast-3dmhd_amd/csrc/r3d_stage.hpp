@@ -1,0 +1,398 @@
+// r3d_stage.hpp - the fused RK3 stage kernel for gfx950 (hydrodynamic path).
+//
+// One kernel launch = one RK stage over the whole domain:
+//     F  = FLUXES(X)                      reference src/fluxes.cpp:44-533
+//     X' = Z + gam_s dt F                 reference src/step.cpp:53-80
+//     Z' = X' + zeta_s dt F               reference src/step.cpp:83-110 (next stage's mix)
+// plus the periodic x/y ghost images of X' (the halo exchange) and, in the
+// last stage, the per-cell dt limits of X' for the next step.
+//
+// Work decomposition.  A workgroup owns a 64 x TY column tile and a chunk of
+// kc planes.  Waves 0..TY+1 each own one row of 64 columns (rows j0-1..j0+TY,
+// the outer two are the y halo); wave TY+2 owns the 2*(TY+2) x-halo columns.
+// Every thread marches up its column in z, holding planes k-1, k, k+1 of its
+// own column in registers (the z stencil) and publishing the plane-k values
+// its x/y neighbours need to a double-buffered LDS plane (one barrier per
+// plane).  Only waves 1..TY compute fluxes; the halo waves only publish.
+//
+// Bit-exactness.  This file must be compiled with -ffp-contract=off: every
+// product and sum below is written in the operand order of the reference
+// CPU sweep it cites, so each intermediate rounds identically.  Quantities the
+// reference computes in a separate sweep over a wider index range (the
+// velocities, P = rho T, the derivative scratch WW1..WW3) are recomputed at
+// the neighbour from the same operands, which yields the same doubles.
+#pragma once
+
+#include "r3d_dev.hpp"
+
+namespace r3d {
+
+// Quantities a column publishes to its in-plane neighbours.
+enum { SH_RU = 0, SH_RV, SH_RW, SH_TT, SH_P, SH_U, SH_V, SH_W, SH_WZ, SH_UZ, SH_VZ, NSH };
+
+struct Raw {
+    double ru, rv, rw, ro, tt;
+};
+
+// Per-plane values of one column (reference src/fluxes.cpp:131-166: P = RO*TT
+// before the inversion, 1/RO, velocities = momentum * (1/RO)); ruw..rww are
+// the vertical advective fluxes m*w of src/fluxes.cpp:183-185.
+struct Col {
+    double ru, rv, rw, ro, tt, p, ri, u, v, w, ruw, rvw, rww;
+};
+
+__device__ __forceinline__ Raw load_raw(const State& s, long n) {
+    Raw r;
+    r.ru = s.q[QRU][n];
+    r.rv = s.q[QRV][n];
+    r.rw = s.q[QRW][n];
+    r.ro = s.q[QRO][n];
+    r.tt = s.q[QTT][n];
+    return r;
+}
+
+__device__ __forceinline__ Col derive(const Raw& r) {
+    Col c;
+    c.ru = r.ru;
+    c.rv = r.rv;
+    c.rw = r.rw;
+    c.ro = r.ro;
+    c.tt = r.tt;
+    c.p = r.ro * r.tt;
+    c.ri = 1.0 / r.ro;
+    c.u = r.ru * c.ri;
+    c.v = r.rv * c.ri;
+    c.w = r.rw * c.ri;
+    c.ruw = c.ru * c.w;
+    c.rvw = c.rv * c.w;
+    c.rww = c.rw * c.w;
+    return c;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Give each XCD a contiguous range of logical tiles, y fastest, so
+// that tiles resident at the same time on one XCD are y-neighbours and share
+// their halo rows through that XCD's L2.  gridDim.x is a multiple of 8.
+__device__ __forceinline__ int logical_block() {
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+__device__ __forceinline__ double dmax(double a, double b) { return a < b ? b : a; }
+__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Per-cell time-step limits of a final state value (reference
+// src/step.cpp:125-209, hydrodynamic branch): pm[0] = max(ogamma*UU/T),
+// pm[1..3] = min of dd/sqrt(fast speed), the thermal and the viscous limit.
+__device__ __forceinline__ void dt_terms(const Consts& C, double ru, double rv, double rw,
+                                         double ro, double tt, double dd, double rk,
+                                         double pm[NPART]) {
+    const double uu = (ru * ru + rv * rv + rw * rw) * (1.0 / ro) * (1.0 / ro);
+    pm[0] = dmax(pm[0], C.og * uu / tt);
+    const double fast = uu + C.gamma * tt + 2.0 * sqrt(uu * C.gamma * tt);
+    pm[1] = dmin(pm[1], dd / sqrt(fast));
+    pm[2] = dmin(pm[2], 0.5 * dd * dd * C.repr * C.cv * ro / rk);
+    pm[3] = dmin(pm[3], 0.375 * dd * dd * C.re * ro);
+}
+
+// Write the block's dt extrema to partial[slot*NPART ...].  red must hold
+// NPART doubles per wave.
+template <int NW>
+__device__ __forceinline__ void block_partials(double pm[NPART], double (*red)[NPART],
+                                               double* partial, int slot) {
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    double v[NPART];
+    v[0] = wave_max(pm[0]);
+    for (int q = 1; q < NPART; ++q) v[q] = wave_min(pm[q]);
+    if (ln == 0)
+        for (int q = 0; q < NPART; ++q) red[wv][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double o[NPART];
+        for (int q = 0; q < NPART; ++q) o[q] = red[0][q];
+        for (int w = 1; w < NW; ++w) {
+            o[0] = dmax(o[0], red[w][0]);
+            for (int q = 1; q < NPART; ++q) o[q] = dmin(o[q], red[w][q]);
+        }
+        for (int q = 0; q < NPART; ++q) partial[(long)slot * NPART + q] = o[q];
+    }
+}
+
+// Store v at (i, j, k) and at its periodic x/y images (the ghost cells the
+// reference's halo exchange, src/communicate.cpp:70-169, fills with it).
+__device__ __forceinline__ void store_with_images(double* f, const Geo& g, int i, int j, long kof,
+                                                  double v) {
+    f[XOFF + i + g.px * j + kof] = v;
+    const int xi = (i == 1) ? g.npx + 1 : (i == g.npx) ? 0 : -1;
+    const int yj = (j == 1) ? g.npy + 1 : (j == g.npy) ? 0 : -1;
+    if (xi >= 0) f[XOFF + xi + g.px * j + kof] = v;
+    if (yj >= 0) f[XOFF + i + g.px * yj + kof] = v;
+    if (xi >= 0 && yj >= 0) f[XOFF + xi + g.px * yj + kof] = v;
+}
+
+template <int TY, int STAGE>
+__global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, const Consts C,
+                                                         const Metrics M) {
+    constexpr int ROWS = TY + 2;
+    constexpr int NW = TY + 3;
+    __shared__ double sh[2][NSH][ROWS][66];
+    __shared__ double red[NW][NPART];
+
+    // Stage 1 takes the dt reduced at the end of the previous step (or by the
+    // standalone dt pass) and makes it the step's dt.
+    const double dt = (STAGE == 0) ? a.scal->dt_next : a.scal->dt;
+    if (STAGE == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.scal->dt = a.scal->dt_next;
+        a.scal->umach = a.scal->umach_next;
+    }
+    const int t = logical_block();
+    if (t >= a.nblocks) return;
+    const int ty = t % a.nty;
+    const int rem = t / a.nty;
+    const int tx = rem % a.ntx;
+    const int kz = rem / a.ntx;
+    const Geo g = a.g;
+    const int i0 = 1 + 64 * tx, j0 = 1 + TY * ty;
+    const int k0 = 1 + a.kc * kz;
+    const int k1 = min(g.npz, k0 + a.kc - 1);
+
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    int r, c, gi, gj;
+    bool act;
+    if (wv < ROWS) {
+        r = wv;
+        c = ln + 1;
+        gi = i0 + ln;
+        gj = j0 - 1 + r;
+        act = true;
+    } else {
+        act = ln < 2 * ROWS;
+        r = act ? (ln >> 1) : 0;
+        c = (ln & 1) ? 65 : 0;
+        gi = (ln & 1) ? i0 + 64 : i0 - 1;
+        gj = j0 - 1 + r;
+    }
+    const bool heavy = (wv >= 1) && (wv <= TY);
+    const bool outp = heavy && gi <= g.npx && gj <= g.npy;
+    const int li = min(gi, g.npx + 1), lj = min(gj, g.npy + 1);
+    const long col = XOFF + li + g.px * lj;
+
+    const double cg = C.gam[STAGE] * dt;
+    const double cz = (STAGE < 2) ? C.zeta[STAGE] * dt : 0.0;
+
+    // x/y metric factors of this column (src/fluxes.cpp:90, 250-254)
+    const double dxx = M.dxx[li], d2xx = M.d2xx[li];
+    const double dyy = M.dyy[lj], d2yy = M.d2yy[lj];
+    const double tmpy = C.hy * dyy;
+    const double tmpy1 = C.hy * d2yy;
+    const double tmpy2 = C.h2y * dyy * dyy;
+    const double ddxy = dmin(M.ddx[li], M.ddy[lj]);
+
+    Col cm = derive(load_raw(a.xin, col + g.pp * (k0 - 1)));
+    Col cc = derive(load_raw(a.xin, col + g.pp * k0));
+    double rw_m2 = a.xin.q[QRW][col + g.pp * max(k0 - 2, 0)];
+    Raw rn = load_raw(a.xin, col + g.pp * min(k0 + 1, g.npz + 1));
+
+    double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
+
+    for (int k = k0; k <= k1; ++k) {
+        const int buf = k & 1;
+        const Raw rnn = load_raw(a.xin, col + g.pp * min(k + 2, g.npz + 1));
+        const Col cp = derive(rn);
+        const double dzz = M.dzz[k], d2zz = M.d2zz[k];
+        // vertical derivatives of this column at plane k
+        // (WW3/WW1/WW2 of src/fluxes.cpp:423, 464, 469)
+        const double wz = (cp.w - cm.w) * C.hz * dzz;
+        const double uz = (cp.u - cm.u) * C.hz * dzz;
+        const double vz = (cp.v - cm.v) * C.hz * dzz;
+        if (act) {
+            sh[buf][SH_RU][r][c] = cc.ru;
+            sh[buf][SH_RV][r][c] = cc.rv;
+            sh[buf][SH_RW][r][c] = cc.rw;
+            sh[buf][SH_TT][r][c] = cc.tt;
+            sh[buf][SH_P][r][c] = cc.p;
+            sh[buf][SH_U][r][c] = cc.u;
+            sh[buf][SH_V][r][c] = cc.v;
+            sh[buf][SH_W][r][c] = cc.w;
+            sh[buf][SH_WZ][r][c] = wz;
+            sh[buf][SH_UZ][r][c] = uz;
+            sh[buf][SH_VZ][r][c] = vz;
+        }
+        __syncthreads();
+
+        if (heavy) {
+#define SH(q, rr, cc_) sh[buf][q][rr][cc_]
+            // One flux component at a time: each block reads only its own LDS
+            // operands, accumulates its terms in the reference's order, applies
+            // the RK update and stores, so that only finished values stay live
+            // (scheduling barriers keep the blocks apart).  Derivatives that
+            // several components need (e.g. d(UU)/dx) are recomputed from the
+            // same operands, which gives the same doubles.
+            const long kof = g.pp * (long)k;
+            const long n = col + kof;
+            const bool bplane = (k == 1) || (k == g.npz);
+            const int zmode = a.zmode;
+            double zc[5];
+            if (STAGE > 0) {
+                for (int f = 0; f < 5; ++f) zc[f] = a.z.q[f][n];
+            } else {
+                zc[0] = cc.ru;  // stage 1: Z = X (save_state, src/step.cpp:38-50)
+                zc[1] = cc.rv;
+                zc[2] = cc.rw;
+                zc[3] = cc.ro;
+                zc[4] = cc.tt;
+            }
+            const double tmpz = C.hz * dzz;
+            const double tmpz1 = C.hz * d2zz;
+            const double tmpz2 = C.h2z * dzz * dzz;
+            const double hxd = C.hx;
+            double xv[5];
+            // RK update of component f (src/step.cpp:53-80), its stores and the
+            // next stage's mix (src/step.cpp:83-110) or the saved state.
+            auto finish = [&](int f, double fval, double cur) {
+                const double x = zc[f] + cg * fval;
+                xv[f] = x;
+                if (outp) {
+                    store_with_images(a.xout.q[f], g, gi, gj, kof, x);
+                    if (zmode == 1) {
+                        a.z.q[f][n] = cur;
+                    } else if (zmode == 0 && STAGE < 2) {
+                        if (!bplane)
+                            a.z.q[f][n] = x + cz * fval;
+                        else  // mixed by the boundary kernel after BCON
+                            a.fb[(k == 1 ? 0 : 5) * g.pp + f * g.pp + XOFF + gi + g.px * gj] = fval;
+                    }
+                }
+            };
+#define LAPX(e, ww, c2) ((e - ww) * hxd * d2xx + (e - c2 + ww) * C.h2x * dxx * dxx)
+            {  // ---- FR: continuity (src/fluxes.cpp:87-121) ----
+                double w1;
+                if (k == 1)
+                    w1 = (4.0 * cp.rw - rnn.rw) * tmpz;
+                else if (k == g.npz)
+                    w1 = (rw_m2 - 4.0 * cm.rw) * tmpz;
+                else
+                    w1 = (cp.rw - cm.rw) * tmpz;
+                double fr = (SH(SH_RU, r, c - 1) - SH(SH_RU, r, c + 1)) * hxd * dxx;
+                fr = fr - (SH(SH_RV, r + 1, c) - SH(SH_RV, r - 1, c)) * tmpy;
+                fr = fr - w1;
+                finish(QRO, fr, cc.ro);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            {  // ---- FU (src/fluxes.cpp:149, 177-185, 289-297, 367, 449, 523) ----
+                const double uE = SH(SH_U, r, c + 1), uW = SH(SH_U, r, c - 1);
+                const double uN = SH(SH_U, r + 1, c), uS = SH(SH_U, r - 1, c);
+                const double u2 = 2.0 * cc.u;
+                double fu = (SH(SH_P, r, c - 1) - SH(SH_P, r, c + 1)) * hxd * dxx;
+                fu = fu - (SH(SH_RU, r, c + 1) * uE - SH(SH_RU, r, c - 1) * uW) * hxd * dxx;
+                fu = fu - (SH(SH_RU, r + 1, c) * SH(SH_V, r + 1, c) -
+                           SH(SH_RU, r - 1, c) * SH(SH_V, r - 1, c)) * tmpy;
+                fu = fu - (cp.ruw - cm.ruw) * tmpz;
+                fu = fu + C.ore * (C.c43 * LAPX(uE, uW, u2) + (uN - uS) * tmpy1 +
+                                   (uN - u2 + uS) * tmpy2 + (cp.u - cm.u) * tmpz1 +
+                                   (cp.u - u2 + cm.u) * tmpz2);
+                const double vxN = (SH(SH_V, r + 1, c + 1) - SH(SH_V, r + 1, c - 1)) * hxd * dxx;
+                const double vxS = (SH(SH_V, r - 1, c + 1) - SH(SH_V, r - 1, c - 1)) * hxd * dxx;
+                fu = fu + C.t2 * (vxN - vxS) * C.hy * dyy;
+                fu = fu + C.t2 * (SH(SH_WZ, r, c + 1) - SH(SH_WZ, r, c - 1)) * hxd * dxx;
+                if (C.lrot) fu = fu + C.omz * cc.rv;
+                finish(QRU, fu, cc.ru);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            {  // ---- FV (src/fluxes.cpp:148, 195-203, 298-307, 373, 455, 528) ----
+                const double vE = SH(SH_V, r, c + 1), vW = SH(SH_V, r, c - 1);
+                const double vN = SH(SH_V, r + 1, c), vS = SH(SH_V, r - 1, c);
+                const double v2 = 2.0 * cc.v;
+                double fv = (SH(SH_P, r - 1, c) - SH(SH_P, r + 1, c)) * tmpy;
+                fv = fv - (SH(SH_RV, r, c + 1) * SH(SH_U, r, c + 1) -
+                           SH(SH_RV, r, c - 1) * SH(SH_U, r, c - 1)) * hxd * dxx;
+                fv = fv - (SH(SH_RV, r + 1, c) * vN - SH(SH_RV, r - 1, c) * vS) * tmpy;
+                fv = fv - (cp.rvw - cm.rvw) * tmpz;
+                fv = fv + C.ore * (LAPX(vE, vW, v2) +
+                                   C.c43 * ((vN - vS) * tmpy1 + (vN - v2 + vS) * tmpy2) +
+                                   (cp.v - cm.v) * tmpz1 + (cp.v - v2 + cm.v) * tmpz2);
+                const double uxN = (SH(SH_U, r + 1, c + 1) - SH(SH_U, r + 1, c - 1)) * hxd * dxx;
+                const double uxS = (SH(SH_U, r - 1, c + 1) - SH(SH_U, r - 1, c - 1)) * hxd * dxx;
+                fv = fv + C.t2 * (uxN - uxS) * C.hy * dyy;
+                fv = fv + C.t2 * (SH(SH_WZ, r + 1, c) - SH(SH_WZ, r - 1, c)) * C.hy * dyy;
+                if (C.lrot) fv = fv + (-C.omz * cc.ru + C.omx * cc.rw);
+                finish(QRV, fv, cc.rv);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            {  // ---- FW (src/fluxes.cpp:129, 147, 213-221, 308-317, 496, 532) ----
+                const double wE = SH(SH_W, r, c + 1), wW = SH(SH_W, r, c - 1);
+                const double wN = SH(SH_W, r + 1, c), wS = SH(SH_W, r - 1, c);
+                const double w2 = 2.0 * cc.w;
+                double fw = C.grav * cc.ro;
+                fw = fw - (cp.p - cm.p) * tmpz;
+                fw = fw - (SH(SH_RW, r, c + 1) * SH(SH_U, r, c + 1) -
+                           SH(SH_RW, r, c - 1) * SH(SH_U, r, c - 1)) * hxd * dxx;
+                fw = fw - (SH(SH_RW, r + 1, c) * SH(SH_V, r + 1, c) -
+                           SH(SH_RW, r - 1, c) * SH(SH_V, r - 1, c)) * tmpy;
+                fw = fw - (cp.rww - cm.rww) * tmpz;
+                fw = fw + C.ore * (LAPX(wE, wW, w2) + (wN - wS) * tmpy1 + (wN - w2 + wS) * tmpy2 +
+                                   C.c43 * ((cp.w - cm.w) * tmpz1 + (cp.w - w2 + cm.w) * tmpz2));
+                fw = fw + C.t2 * ((SH(SH_UZ, r, c + 1) - SH(SH_UZ, r, c - 1)) * hxd * dxx +
+                                  (SH(SH_VZ, r + 1, c) - SH(SH_VZ, r - 1, c)) * C.hy * dyy);
+                if (C.lrot) fw = fw + -C.omx * cc.rv;
+                finish(QRW, fw, cc.rw);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            {  // ---- FT (src/fluxes.cpp:235-272, 349-512) ----
+                const double rk = M.rkapa[k], dk = M.dkapa[k];
+                const double tt0 = cc.tt, ttU = cp.tt, ttD = cm.tt;
+                const double ttE = SH(SH_TT, r, c + 1), ttW = SH(SH_TT, r, c - 1);
+                const double ttN = SH(SH_TT, r + 1, c), ttS = SH(SH_TT, r - 1, c);
+                const double t2c = 2.0 * tt0;
+                double ft = -1.0 * cc.u * (ttE - ttW) * hxd * dxx;
+                ft = ft - cc.v * (ttN - ttS) * tmpy;
+                ft = ft - cc.w * (ttU - ttD) * tmpz;
+                ft = ft + ((ttE - ttW) * hxd * d2xx + (ttE - t2c + ttW) * C.h2x * dxx * dxx) *
+                              cc.ri * C.tk * rk;
+                ft = ft + ((ttN - ttS) * tmpy1 + (ttN - t2c + ttS) * tmpy2) * cc.ri * C.tk * rk;
+                ft = ft + ((ttU - ttD) * tmpz1 + (ttU - t2c + ttD) * tmpz2) * cc.ri * C.tk * rk;
+                ft = ft + (ttU - ttD) * C.hz * dzz * cc.ri * C.tk * dk;
+                const double ux = (SH(SH_U, r, c + 1) - SH(SH_U, r, c - 1)) * hxd * dxx;
+                const double vx = (SH(SH_V, r, c + 1) - SH(SH_V, r, c - 1)) * hxd * dxx;
+                const double wx = (SH(SH_W, r, c + 1) - SH(SH_W, r, c - 1)) * hxd * dxx;
+                ft = ft + (C.c43 * ux * ux + vx * vx + wx * wx) * cc.ri * C.t1;
+                ft = ft - C.ocv * ux * tt0;
+                const double uy = (SH(SH_U, r + 1, c) - SH(SH_U, r - 1, c)) * C.hy * dyy;
+                ft = ft + (uy * uy + 2.0 * uy * vx) * cc.ri * C.t1;
+                const double vy = (SH(SH_V, r + 1, c) - SH(SH_V, r - 1, c)) * C.hy * dyy;
+                ft = ft + vy * vy * cc.ri * C.t3;
+                ft = ft - C.ocv * vy * tt0;
+                ft = ft + (wz * wz - ux * wz - ux * vy - vy * wz) * cc.ri * C.t3;
+                ft = ft - C.ocv * wz * tt0;
+                const double wy = (SH(SH_W, r + 1, c) - SH(SH_W, r - 1, c)) * C.hy * dyy;
+                ft = ft + (uz * uz + vz * vz + wy * wy + 2.0 * vz * wy) * cc.ri * C.t1;
+                ft = ft + 2.0 * uz * wx * cc.ri * C.t1;
+                finish(QTT, ft, cc.tt);
+            }
+#undef LAPX
+#undef SH
+            if (STAGE == 2 && a.want_dt && outp && !bplane) {
+                const double dd = dmin(ddxy, M.ddz[k]);
+                dt_terms(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], dd, M.rkapa[k], pm);
+            }
+        }
+
+        rw_m2 = cm.rw;
+        cm = cc;
+        cc = cp;
+        rn = rnn;
+    }
+
+    if (STAGE == 2 && a.want_dt) block_partials<NW>(pm, red, a.partial, t);
+}
+
+}  // namespace r3d

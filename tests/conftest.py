@@ -18,3 +18,17 @@ def golden():
     import json
     with open(os.path.join(ROOT, "tests", "golden", "ref_hashes.json")) as f:
         return json.load(f)
+
+
+def load_package():
+    """Import the rast-3dmhd_amd package (its directory name is not an identifier)."""
+    import importlib.util
+    if "rast3dmhd_amd" in sys.modules:
+        return sys.modules["rast3dmhd_amd"]
+    path = os.path.join(ROOT, "rast-3dmhd_amd", "__init__.py")
+    spec = importlib.util.spec_from_file_location(
+        "rast3dmhd_amd", path, submodule_search_locations=[os.path.dirname(path)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["rast3dmhd_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
