@@ -34,7 +34,7 @@ int r3d_host_cfg(const r3d_host* h, r3d_hip_cfg* out);
 /* Global metrics (pointers stay valid while h lives). */
 int r3d_host_metrics(const r3d_host* h, r3d_hip_metrics* out);
 /* Global initial state: fields[R3D_RU..R3D_BZ], each (npx+2)(npy+2)(npz+2)
- * doubles, x fastest (B entries are zero-filled arrays when lmag is off). */
+ * doubles, x fastest (B entries are NULL when lmag is off). */
 int r3d_host_state(const r3d_host* h, const double* fields[8]);
 /* dims = {npx, npy, npz, npey, npez, lmag, ntotal, nstep0}. */
 void r3d_host_dims(const r3d_host* h, int dims[8]);

@@ -285,12 +285,20 @@ class Run:
             out[name] = np.ctypeslib.as_array(getattr(m, name), shape=(ln,)).copy()
         return out
 
-    def state(self):
-        """Global initial primaries, (npz+2, npy+2, npx+2) each (copies)."""
+    def state(self, copy=True):
+        """Global initial primaries, (npz+2, npy+2, npx+2) each.
+
+        copy=False returns views of the host library's arrays (valid while
+        this Run lives), which avoids doubling host memory on large grids."""
         ptrs = (ctypes.POINTER(ctypes.c_double) * 8)()
         load_host().r3d_host_state(self.h, ptrs)
-        return {name: np.ctypeslib.as_array(ptrs[q], shape=self.shape).copy()
-                for q, name in enumerate(PRIMARIES)}
+        out = {}
+        for q, name in enumerate(PRIMARIES):
+            if not ptrs[q]:
+                continue  # B fields without LMAG
+            a = np.ctypeslib.as_array(ptrs[q], shape=self.shape)
+            out[name] = a.copy() if copy else a
+        return out
 
     def tb(self):
         return load_host().r3d_host_tb(self.h)
@@ -323,6 +331,6 @@ class Run:
         cfg.variant = variant
         sim = HipSim(cfg)
         sim.set_metrics(self.metrics())
-        sim.upload_global(self.state())
+        sim.upload_global(self.state(copy=False))
         sim.set_scalars()
         return sim

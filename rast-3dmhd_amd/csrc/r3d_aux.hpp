@@ -2,11 +2,12 @@
 //
 // k_bound   : per stage, after k_stage.  BCON on the global top (k = 1) and
 //             bottom (k = npz) planes (reference src/bcon.cpp:12-141), the
-//             ghost images of the rewritten values, the deferred Z' mix on
-//             those planes, their dt limits in the last stage, and the
+//             deferred Z' mix on those planes, their dt limits in the last
+//             stage, and the
 //             1/RO flip of the two outer ghost planes that the reference's
 //             FLUXES performs on its full array (src/fluxes.cpp:133-138) but
 //             its halo exchange never refreshes.
+// k_halo    : per stage, after k_bound: periodic x/y ghost images.
 // k_dt      : dt limits of the current state when no fused value is
 //             available (first step, after an upload or a truncated step).
 // k_finish  : reduces the dt partials and advances the step counters.
@@ -38,7 +39,6 @@ __global__ __launch_bounds__(BOUND_THREADS) void k_bound(const StageArgs a, cons
 
         if (i >= 1 && i <= g.npx && j >= 1 && j <= g.npy) {
             const int kb = side ? g.npz : 1;
-            const long kof = g.pp * (long)kb;
             const long n = g.at(i, j, kb);
             const long d1 = side ? -g.pp : g.pp;
             double* const* X = a.xout.q;
@@ -74,10 +74,10 @@ __global__ __launch_bounds__(BOUND_THREADS) void k_bound(const StageArgs a, cons
                 else
                     tt = C.c43 * tt1 - C.c13 * tt2 + C.c23 * M.dz_bot * C.theta / M.rkapa[kb];
             }
-            store_with_images(X[QRU], g, i, j, kof, ru);
-            store_with_images(X[QRV], g, i, j, kof, rv);
-            store_with_images(X[QRW], g, i, j, kof, rw);
-            store_with_images(X[QTT], g, i, j, kof, tt);
+            X[QRU][n] = ru;
+            X[QRV][n] = rv;
+            X[QRW][n] = rw;
+            X[QTT][n] = tt;
 
             if (a.zmode == 0 && a.stage < 2) {
                 // mix for the next stage with the post-BCON state
@@ -118,6 +118,31 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt(const Geo g, const State s, c
                  M.rkapa[k], pm);
     }
     block_partials<DT_THREADS / 64>(pm, red, partial, blockIdx.x);
+}
+
+// Periodic x/y ghost images of X' on planes 1..npz (the reference's halo
+// exchange, src/communicate.cpp:70-169, for one global domain): every ghost
+// cell takes the interior cell it wraps to.  grid = (perimeter blocks, npz).
+constexpr int HALO_THREADS = 256;
+__global__ __launch_bounds__(HALO_THREADS) void k_halo(const Geo g, const State x, int nf) {
+    const int k = 1 + blockIdx.y;
+    const int W = g.npx + 2;
+    const int q = blockIdx.x * HALO_THREADS + threadIdx.x;
+    int i, j;
+    if (q < 2 * W) {  // ghost rows j = 0 and j = npy+1, all i
+        i = q % W;
+        j = q < W ? 0 : g.npy + 1;
+    } else if (q < 2 * W + 2 * g.npy) {  // ghost columns i = 0 and i = npx+1, interior j
+        const int m = q - 2 * W;
+        j = 1 + m % g.npy;
+        i = m < g.npy ? 0 : g.npx + 1;
+    } else {
+        return;
+    }
+    const int si = i == 0 ? g.npx : i == g.npx + 1 ? 1 : i;
+    const int sj = j == 0 ? g.npy : j == g.npy + 1 ? 1 : j;
+    const long d = g.at(i, j, k), s = g.at(si, sj, k);
+    for (int f = 0; f < nf; ++f) x.q[f][d] = x.q[f][s];
 }
 
 // One block of 1024 threads.  reduce: fold partial[0..npart) into dt_next /

@@ -229,17 +229,25 @@ int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     const StageArgs a = stage_args(c, stage, zmode, want_dt);
     const dim3 blk((TILE_Y + 3) * 64);
     int rc = timed(c, 0, [&] {
-        if (stage == 0)
-            hipLaunchKernelGGL((k_stage<TILE_Y, 0>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M);
-        else if (stage == 1)
-            hipLaunchKernelGGL((k_stage<TILE_Y, 1>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M);
-        else
-            hipLaunchKernelGGL((k_stage<TILE_Y, 2>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M);
+#define R3D_STAGE(S, Z) \
+    hipLaunchKernelGGL((k_stage<TILE_Y, S, Z>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
+        if (stage == 0 && zmode == 0) R3D_STAGE(0, 0);
+        else if (stage == 0) R3D_STAGE(0, 1);
+        else if (stage == 1 && zmode == 0) R3D_STAGE(1, 0);
+        else if (stage == 1) R3D_STAGE(1, 2);
+        else R3D_STAGE(2, 2);
+#undef R3D_STAGE
     });
     if (rc) return rc;
     rc = timed(c, 1, [&] {
         hipLaunchKernelGGL(k_bound, dim3(c->nbb, 2), dim3(BOUND_THREADS), 0, c->stream, a, c->C,
                            c->M, c->nblocks);
+    });
+    if (rc) return rc;
+    rc = timed(c, 3, [&] {
+        const int per = 2 * (c->cfg.npx + 2) + 2 * c->cfg.npy;
+        hipLaunchKernelGGL(k_halo, dim3((per + HALO_THREADS - 1) / HALO_THREADS, c->cfg.npz),
+                           dim3(HALO_THREADS), 0, c->stream, c->g, a.xout, c->nf);
     });
     if (rc) return rc;
     c->cur ^= 1;

@@ -127,19 +127,9 @@ __device__ __forceinline__ void block_partials(double pm[NPART], double (*red)[N
     }
 }
 
-// Store v at (i, j, k) and at its periodic x/y images (the ghost cells the
-// reference's halo exchange, src/communicate.cpp:70-169, fills with it).
-__device__ __forceinline__ void store_with_images(double* f, const Geo& g, int i, int j, long kof,
-                                                  double v) {
-    f[XOFF + i + g.px * j + kof] = v;
-    const int xi = (i == 1) ? g.npx + 1 : (i == g.npx) ? 0 : -1;
-    const int yj = (j == 1) ? g.npy + 1 : (j == g.npy) ? 0 : -1;
-    if (xi >= 0) f[XOFF + xi + g.px * j + kof] = v;
-    if (yj >= 0) f[XOFF + i + g.px * yj + kof] = v;
-    if (xi >= 0 && yj >= 0) f[XOFF + xi + g.px * yj + kof] = v;
-}
-
-template <int TY, int STAGE>
+// ZMODE: 0 = Z' is the next stage's mix, 1 = Z keeps the saved state (a step
+// truncated after stage 1), 2 = Z is not written (last stage of the step).
+template <int TY, int STAGE, int ZMODE>
 __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, const Consts C,
                                                          const Metrics M) {
     constexpr int ROWS = TY + 2;
@@ -192,10 +182,7 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
     // x/y metric factors of this column (src/fluxes.cpp:90, 250-254)
     const double dxx = M.dxx[li], d2xx = M.d2xx[li];
     const double dyy = M.dyy[lj], d2yy = M.d2yy[lj];
-    const double tmpy = C.hy * dyy;
-    const double tmpy1 = C.hy * d2yy;
-    const double tmpy2 = C.h2y * dyy * dyy;
-    const double ddxy = dmin(M.ddx[li], M.ddy[lj]);
+    const double ddxy = (STAGE == 2) ? dmin(M.ddx[li], M.ddy[lj]) : 0.0;
 
     Col cm = derive(load_raw(a.xin, col + g.pp * (k0 - 1)));
     Col cc = derive(load_raw(a.xin, col + g.pp * k0));
@@ -240,7 +227,10 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
             const long kof = g.pp * (long)k;
             const long n = col + kof;
             const bool bplane = (k == 1) || (k == g.npz);
-            const int zmode = a.zmode;
+            // y metric products, recomputed per plane (cheaper than holding them)
+            const double tmpy = C.hy * dyy;
+            const double tmpy1 = C.hy * d2yy;
+            const double tmpy2 = C.h2y * dyy * dyy;
             double zc[5];
             if (STAGE > 0) {
                 for (int f = 0; f < 5; ++f) zc[f] = a.z.q[f][n];
@@ -257,15 +247,16 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
             const double hxd = C.hx;
             double xv[5];
             // RK update of component f (src/step.cpp:53-80), its stores and the
-            // next stage's mix (src/step.cpp:83-110) or the saved state.
+            // next stage's mix (src/step.cpp:83-110) or the saved state.  The
+            // periodic ghost images are written afterwards by k_halo.
             auto finish = [&](int f, double fval, double cur) {
                 const double x = zc[f] + cg * fval;
                 xv[f] = x;
                 if (outp) {
-                    store_with_images(a.xout.q[f], g, gi, gj, kof, x);
-                    if (zmode == 1) {
+                    a.xout.q[f][n] = x;
+                    if (ZMODE == 1) {
                         a.z.q[f][n] = cur;
-                    } else if (zmode == 0 && STAGE < 2) {
+                    } else if (ZMODE == 0 && STAGE < 2) {
                         if (!bplane)
                             a.z.q[f][n] = x + cz * fval;
                         else  // mixed by the boundary kernel after BCON

@@ -464,7 +464,8 @@ int build_state(r3d_host* h) {
     const Params& p = h->p;
     const int X = p.npx + 2, Y = p.npy + 2, Z = p.npz + 2;
     const size_t plane = (size_t)X * Y, n = plane * Z;
-    for (int q = 0; q < 8; ++q) h->f[q].assign(n, 0.0);
+    const int nf = p.lmag ? 8 : 5;  // B fields only exist with LMAG
+    for (int q = 0; q < nf; ++q) h->f[q].assign(n, 0.0);
     std::vector<double> T, R;
     Profile prof(p);
     if (prof.build(p.npz, T, R)) return -1;
@@ -505,7 +506,7 @@ int build_state(r3d_host* h) {
                 }
     }
     // periodic x/y images of every plane (src/communicate.cpp:105-168)
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < nf; ++q) {
         double* a = h->f[q].data();
         for (int k = 0; k < Z; ++k) {
             for (int i = 0; i < X; ++i) {
@@ -630,7 +631,7 @@ int r3d_host_metrics(const r3d_host* h, r3d_hip_metrics* o) {
 
 int r3d_host_state(const r3d_host* h, const double* fields[8]) {
     if (!h || !fields) return fail("null argument");
-    for (int q = 0; q < 8; ++q) fields[q] = h->f[q].data();
+    for (int q = 0; q < 8; ++q) fields[q] = h->f[q].empty() ? nullptr : h->f[q].data();
     return 0;
 }
 
