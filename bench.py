@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--npey", type=int, default=4)
     ap.add_argument("--npez", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calib-gib", type=float, default=0.0,
+                    help="before the run, time one device-to-device copy of this many GiB "
+                         "(a plain copy of known size, used to scale profiler byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -110,6 +113,17 @@ def main():
         sys.exit(2)
 
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.npx, a.npy, a.npey, a.npez, CPU_CORES)
+
+    if a.copy_calib_gib > 0:
+        import torch
+        n = int(a.copy_calib_gib * 2**30 / 8)
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: {n * 8} bytes copied once")
+        del src, dst
+        torch.cuda.empty_cache()
 
     pkg = load_package()
     args = ["--npx", str(a.npx), "--npy", str(a.npy), "--npz", str(a.npz),

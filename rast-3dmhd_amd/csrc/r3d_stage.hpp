@@ -41,13 +41,27 @@ struct Col {
     double ru, rv, rw, ro, tt, p, ri, u, v, w, ruw, rvw, rww;
 };
 
-__device__ __forceinline__ Raw load_raw(const State& s, long n) {
+// Global accesses of the stage kernel use a plane base that is uniform over
+// the workgroup (field pointer + k * plane bytes, held in scalar registers)
+// plus one 32-bit per-lane byte offset within the plane, so that no per-lane
+// 64-bit address has to stay live across the z loop.
+__device__ __forceinline__ const char* plane_base(const double* f, long pp, int k) {
+    return reinterpret_cast<const char*>(f) + (size_t)pp * 8u * (size_t)k;
+}
+__device__ __forceinline__ double ldp(const double* f, long pp, int k, uint32_t voff) {
+    return *reinterpret_cast<const double*>(plane_base(f, pp, k) + voff);
+}
+__device__ __forceinline__ void stp(double* f, long pp, int k, uint32_t voff, double v) {
+    *reinterpret_cast<double*>(const_cast<char*>(plane_base(f, pp, k)) + voff) = v;
+}
+
+__device__ __forceinline__ Raw load_raw(const State& s, long pp, int k, uint32_t voff) {
     Raw r;
-    r.ru = s.q[QRU][n];
-    r.rv = s.q[QRV][n];
-    r.rw = s.q[QRW][n];
-    r.ro = s.q[QRO][n];
-    r.tt = s.q[QTT][n];
+    r.ru = ldp(s.q[QRU], pp, k, voff);
+    r.rv = ldp(s.q[QRV], pp, k, voff);
+    r.rw = ldp(s.q[QRW], pp, k, voff);
+    r.ro = ldp(s.q[QRO], pp, k, voff);
+    r.tt = ldp(s.q[QTT], pp, k, voff);
     return r;
 }
 
@@ -136,6 +150,11 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
     constexpr int NW = TY + 3;
     __shared__ double sh[2][NSH][ROWS][66];
     __shared__ double red[NW][NPART];
+    // Per-plane z metrics of this block's chunk, staged once: they are
+    // uniform per plane, and fetching them inside the z loop would put a
+    // global-memory round trip on every plane.
+    constexpr int KCMAX = 128;
+    __shared__ double zm[5][KCMAX];  // dzz, d2zz, ddz, rkapa, dkapa
 
     // Stage 1 takes the dt reduced at the end of the previous step (or by the
     // standalone dt pass) and makes it the step's dt.
@@ -174,7 +193,17 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
     const bool heavy = (wv >= 1) && (wv <= TY);
     const bool outp = heavy && gi <= g.npx && gj <= g.npy;
     const int li = min(gi, g.npx + 1), lj = min(gj, g.npy + 1);
-    const long col = XOFF + li + g.px * lj;
+    // byte offset of this column within a plane (a plane is < 4 GiB)
+    uint32_t voff = (uint32_t)((XOFF + li + (long)g.px * lj) * 8);
+
+    for (int q = threadIdx.x; q < 5 * KCMAX; q += blockDim.x) {
+        const int f = q / KCMAX, kk = k0 + q % KCMAX;
+        if (kk <= k1) {
+            const double* src = f == 0 ? M.dzz : f == 1 ? M.d2zz : f == 2 ? M.ddz : f == 3 ? M.rkapa : M.dkapa;
+            zm[f][kk - k0] = src[kk];
+        }
+    }
+    __syncthreads();
 
     const double cg = C.gam[STAGE] * dt;
     const double cz = (STAGE < 2) ? C.zeta[STAGE] * dt : 0.0;
@@ -184,18 +213,26 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
     const double dyy = M.dyy[lj], d2yy = M.d2yy[lj];
     const double ddxy = (STAGE == 2) ? dmin(M.ddx[li], M.ddy[lj]) : 0.0;
 
-    Col cm = derive(load_raw(a.xin, col + g.pp * (k0 - 1)));
-    Col cc = derive(load_raw(a.xin, col + g.pp * k0));
-    double rw_m2 = a.xin.q[QRW][col + g.pp * max(k0 - 2, 0)];
-    Raw rn = load_raw(a.xin, col + g.pp * min(k0 + 1, g.npz + 1));
+    Col cm = derive(load_raw(a.xin, g.pp, k0 - 1, voff));
+    Col cc = derive(load_raw(a.xin, g.pp, k0, voff));
+    double rw_m2 = ldp(a.xin.q[QRW], g.pp, max(k0 - 2, 0), voff);
+    Raw rn = load_raw(a.xin, g.pp, min(k0 + 1, g.npz + 1), voff);
 
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
 
     for (int k = k0; k <= k1; ++k) {
         const int buf = k & 1;
-        const Raw rnn = load_raw(a.xin, col + g.pp * min(k + 2, g.npz + 1));
+        // Keep the lane offset opaque to loop-invariant code motion: every
+        // access is then (uniform plane base) + zext(voff), the saddr form,
+        // instead of one hoisted 64-bit per-lane address per field.
+        asm volatile("" : "+v"(voff));
+        const Raw rnn = load_raw(a.xin, g.pp, min(k + 2, g.npz + 1), voff);
+        double zc[5];
+        if (STAGE > 0 && heavy)
+            for (int f = 0; f < 5; ++f) zc[f] = ldp(a.z.q[f], g.pp, k, voff);
         const Col cp = derive(rn);
-        const double dzz = M.dzz[k], d2zz = M.d2zz[k];
+        const int kl = k - k0;
+        const double dzz = zm[0][kl], d2zz = zm[1][kl];
         // vertical derivatives of this column at plane k
         // (WW3/WW1/WW2 of src/fluxes.cpp:423, 464, 469)
         const double wz = (cp.w - cm.w) * C.hz * dzz;
@@ -224,17 +261,12 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
             // (scheduling barriers keep the blocks apart).  Derivatives that
             // several components need (e.g. d(UU)/dx) are recomputed from the
             // same operands, which gives the same doubles.
-            const long kof = g.pp * (long)k;
-            const long n = col + kof;
             const bool bplane = (k == 1) || (k == g.npz);
             // y metric products, recomputed per plane (cheaper than holding them)
             const double tmpy = C.hy * dyy;
             const double tmpy1 = C.hy * d2yy;
             const double tmpy2 = C.h2y * dyy * dyy;
-            double zc[5];
-            if (STAGE > 0) {
-                for (int f = 0; f < 5; ++f) zc[f] = a.z.q[f][n];
-            } else {
+            if (STAGE == 0) {
                 zc[0] = cc.ru;  // stage 1: Z = X (save_state, src/step.cpp:38-50)
                 zc[1] = cc.rv;
                 zc[2] = cc.rw;
@@ -253,14 +285,14 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                 const double x = zc[f] + cg * fval;
                 xv[f] = x;
                 if (outp) {
-                    a.xout.q[f][n] = x;
+                    stp(a.xout.q[f], g.pp, k, voff, x);
                     if (ZMODE == 1) {
-                        a.z.q[f][n] = cur;
+                        stp(a.z.q[f], g.pp, k, voff, cur);
                     } else if (ZMODE == 0 && STAGE < 2) {
                         if (!bplane)
-                            a.z.q[f][n] = x + cz * fval;
+                            stp(a.z.q[f], g.pp, k, voff, x + cz * fval);
                         else  // mixed by the boundary kernel after BCON
-                            a.fb[(k == 1 ? 0 : 5) * g.pp + f * g.pp + XOFF + gi + g.px * gj] = fval;
+                            stp(a.fb, g.pp, (k == 1 ? 0 : 5) + f, voff, fval);
                     }
                 }
             };
@@ -339,7 +371,7 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
             }
             __builtin_amdgcn_sched_barrier(0);
             {  // ---- FT (src/fluxes.cpp:235-272, 349-512) ----
-                const double rk = M.rkapa[k], dk = M.dkapa[k];
+                const double rk = zm[3][kl], dk = zm[4][kl];
                 const double tt0 = cc.tt, ttU = cp.tt, ttD = cm.tt;
                 const double ttE = SH(SH_TT, r, c + 1), ttW = SH(SH_TT, r, c - 1);
                 const double ttN = SH(SH_TT, r + 1, c), ttS = SH(SH_TT, r - 1, c);
@@ -372,8 +404,8 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
 #undef LAPX
 #undef SH
             if (STAGE == 2 && a.want_dt && outp && !bplane) {
-                const double dd = dmin(ddxy, M.ddz[k]);
-                dt_terms(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], dd, M.rkapa[k], pm);
+                const double dd = dmin(ddxy, zm[2][kl]);
+                dt_terms(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], dd, zm[3][kl], pm);
             }
         }
 
