@@ -279,37 +279,39 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
             const double hxd = C.hx;
             double xv[5];
             // RK update of component f (src/step.cpp:53-80), its stores and the
-            // next stage's mix (src/step.cpp:83-110) or the saved state.  The
-            // periodic ghost images are written afterwards by k_halo.
+            // next stage's mix (src/step.cpp:83-110) or the saved state.
+            // The stores are unconditional so that the five components form
+            // one basic block the scheduler can interleave: a heavy lane that
+            // is not an output cell (past npx or npy in an edge tile) has its
+            // offset clamped to the ghost column npx+1 / ghost row npy+1, and
+            // whatever it writes there is replaced by k_halo's periodic
+            // images (X') or never read (Z, F on ghosts).
             auto finish = [&](int f, double fval, double cur) {
                 const double x = zc[f] + cg * fval;
                 xv[f] = x;
-                if (outp) {
-                    stp(a.xout.q[f], g.pp, k, voff, x);
-                    if (ZMODE == 1) {
-                        stp(a.z.q[f], g.pp, k, voff, cur);
-                    } else if (ZMODE == 0 && STAGE < 2) {
-                        if (!bplane)
-                            stp(a.z.q[f], g.pp, k, voff, x + cz * fval);
-                        else  // mixed by the boundary kernel after BCON
-                            stp(a.fb, g.pp, (k == 1 ? 0 : 5) + f, voff, fval);
-                    }
+                stp(a.xout.q[f], g.pp, k, voff, x);
+                if (ZMODE == 1) {
+                    stp(a.z.q[f], g.pp, k, voff, cur);
+                } else if (ZMODE == 0 && STAGE < 2) {
+                    // on the two BCON planes F is parked for the boundary
+                    // kernel, which mixes after BCON (uniform select)
+                    double* dst = bplane ? a.fb : a.z.q[f];
+                    const int kp = bplane ? (k == 1 ? 0 : 5) + f : k;
+                    stp(dst, g.pp, kp, voff, bplane ? fval : x + cz * fval);
                 }
             };
 #define LAPX(e, ww, c2) ((e - ww) * hxd * d2xx + (e - c2 + ww) * C.h2x * dxx * dxx)
             {  // ---- FR: continuity (src/fluxes.cpp:87-121) ----
-                double w1;
-                if (k == 1)
-                    w1 = (4.0 * cp.rw - rnn.rw) * tmpz;
-                else if (k == g.npz)
-                    w1 = (rw_m2 - 4.0 * cm.rw) * tmpz;
-                else
-                    w1 = (cp.rw - cm.rw) * tmpz;
+                const double w1top = (4.0 * cp.rw - rnn.rw) * tmpz;  // one-sided, top
+                const double w1bot = (rw_m2 - 4.0 * cm.rw) * tmpz;   // one-sided, bottom
+                const double w1mid = (cp.rw - cm.rw) * tmpz;
+                const double w1 = k == 1 ? w1top : (k == g.npz ? w1bot : w1mid);
                 double fr = (SH(SH_RU, r, c - 1) - SH(SH_RU, r, c + 1)) * hxd * dxx;
                 fr = fr - (SH(SH_RV, r + 1, c) - SH(SH_RV, r - 1, c)) * tmpy;
                 fr = fr - w1;
                 finish(QRO, fr, cc.ro);
             }
+            
             __builtin_amdgcn_sched_barrier(0);
             {  // ---- FU (src/fluxes.cpp:149, 177-185, 289-297, 367, 449, 523) ----
                 const double uE = SH(SH_U, r, c + 1), uW = SH(SH_U, r, c - 1);
@@ -330,6 +332,7 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                 if (C.lrot) fu = fu + C.omz * cc.rv;
                 finish(QRU, fu, cc.ru);
             }
+            
             __builtin_amdgcn_sched_barrier(0);
             {  // ---- FV (src/fluxes.cpp:148, 195-203, 298-307, 373, 455, 528) ----
                 const double vE = SH(SH_V, r, c + 1), vW = SH(SH_V, r, c - 1);
@@ -350,6 +353,7 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                 if (C.lrot) fv = fv + (-C.omz * cc.ru + C.omx * cc.rw);
                 finish(QRV, fv, cc.rv);
             }
+            
             __builtin_amdgcn_sched_barrier(0);
             {  // ---- FW (src/fluxes.cpp:129, 147, 213-221, 308-317, 496, 532) ----
                 const double wE = SH(SH_W, r, c + 1), wW = SH(SH_W, r, c - 1);
@@ -369,6 +373,7 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                 if (C.lrot) fw = fw + -C.omx * cc.rv;
                 finish(QRW, fw, cc.rw);
             }
+            
             __builtin_amdgcn_sched_barrier(0);
             {  // ---- FT (src/fluxes.cpp:235-272, 349-512) ----
                 const double rk = zm[3][kl], dk = zm[4][kl];
@@ -403,9 +408,14 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
             }
 #undef LAPX
 #undef SH
-            if (STAGE == 2 && a.want_dt && outp && !bplane) {
+            if (STAGE == 2 && a.want_dt) {
+                // evaluated on every lane, folded in only on output cells
+                double q[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
                 const double dd = dmin(ddxy, zm[2][kl]);
-                dt_terms(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], dd, zm[3][kl], pm);
+                dt_terms(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], dd, zm[3][kl], q);
+                const bool use = outp && !bplane;
+                pm[0] = use ? dmax(pm[0], q[0]) : pm[0];
+                for (int v = 1; v < NPART; ++v) pm[v] = use ? dmin(pm[v], q[v]) : pm[v];
             }
         }
 
