@@ -1,7 +1,8 @@
 // r3d_aux.hpp - boundary-plane, dt and bookkeeping kernels (gfx950).
 //
 // k_bound   : per stage, after k_stage.  BCON on the global top (k = 1) and
-//             bottom (k = npz) planes (reference src/bcon.cpp:12-141), the
+//             bottom (k = npz) planes (reference src/bcon.cpp:12-141, with
+//             BZ = 0 under LMAG), the
 //             deferred Z' mix on those planes, their dt limits in the last
 //             stage, and the
 //             1/RO flip of the two outer ghost planes that the reference's
@@ -78,19 +79,37 @@ __global__ __launch_bounds__(BOUND_THREADS) void k_bound(const StageArgs a, cons
             X[QRV][n] = rv;
             X[QRW][n] = rw;
             X[QTT][n] = tt;
+            // LMAG: BZ = 0 on both walls (src/bcon.cpp:95-98, 136-139); BX, BY
+            // keep their updated values
+            double bx = 0.0, by = 0.0, bz = 0.0;
+            if (C.lmag) {
+                bx = X[QBX][n];
+                by = X[QBY][n];
+                bz = C.bz_zero ? 0.0 : X[QBZ][n];
+                X[QBZ][n] = bz;
+            }
 
             if (a.zmode == 0 && a.stage < 2) {
                 // mix for the next stage with the post-BCON state
-                const long fo = (side ? 5 : 0) * g.pp + XOFF + i + g.px * j;
+                const int nf = C.lmag ? 8 : 5;
+                const long fo = (side ? nf : 0) * g.pp + XOFF + i + g.px * j;
                 a.z.q[QRU][n] = ru + cz * a.fb[fo + 0 * g.pp];
                 a.z.q[QRV][n] = rv + cz * a.fb[fo + 1 * g.pp];
                 a.z.q[QRW][n] = rw + cz * a.fb[fo + 2 * g.pp];
                 a.z.q[QRO][n] = ro + cz * a.fb[fo + 3 * g.pp];
                 a.z.q[QTT][n] = tt + cz * a.fb[fo + 4 * g.pp];
+                if (C.lmag) {
+                    a.z.q[QBX][n] = bx + cz * a.fb[fo + 5 * g.pp];
+                    a.z.q[QBY][n] = by + cz * a.fb[fo + 6 * g.pp];
+                    a.z.q[QBZ][n] = bz + cz * a.fb[fo + 7 * g.pp];
+                }
             }
             if (a.stage == 2 && a.want_dt) {
                 const double dd = dmin(dmin(M.ddx[i], M.ddy[j]), M.ddz[kb]);
-                dt_terms(C, ru, rv, rw, ro, tt, dd, M.rkapa[kb], pm);
+                if (C.lmag)
+                    dt_terms<true>(C, ru, rv, rw, ro, tt, bx, by, bz, dd, M.rkapa[kb], pm);
+                else
+                    dt_terms<false>(C, ru, rv, rw, ro, tt, 0.0, 0.0, 0.0, dd, M.rkapa[kb], pm);
             }
         }
     }
@@ -114,8 +133,12 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt(const Geo g, const State s, c
         const int k = 1 + (int)(rest / g.npy);
         const long n = g.at(i, j, k);
         const double dd = dmin(dmin(M.ddx[i], M.ddy[j]), M.ddz[k]);
-        dt_terms(C, s.q[QRU][n], s.q[QRV][n], s.q[QRW][n], s.q[QRO][n], s.q[QTT][n], dd,
-                 M.rkapa[k], pm);
+        if (C.lmag)
+            dt_terms<true>(C, s.q[QRU][n], s.q[QRV][n], s.q[QRW][n], s.q[QRO][n], s.q[QTT][n],
+                           s.q[QBX][n], s.q[QBY][n], s.q[QBZ][n], dd, M.rkapa[k], pm);
+        else
+            dt_terms<false>(C, s.q[QRU][n], s.q[QRV][n], s.q[QRW][n], s.q[QRO][n], s.q[QTT][n],
+                            0.0, 0.0, 0.0, dd, M.rkapa[k], pm);
     }
     block_partials<DT_THREADS / 64>(pm, red, partial, blockIdx.x);
 }

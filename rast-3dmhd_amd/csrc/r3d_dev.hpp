@@ -51,9 +51,12 @@ struct Consts {
     double t3;   // c43*ocv*ore       (src/fluxes.cpp:399)
     double og;   // 1/gamma           (src/step.cpp:119)
     double gamma, re, repr, cv, sff, rm, obeta, orm;
+    double ohm;  // obeta*orm*ocv     (golden/fortran/3dmhdsub.f:2307)
     double theta, tb, hh, tpr, tp;
     double gam[3], zeta[2];
     int lrot, lmag, top_mode, half_t, izcon, bz_zero;
+    int mag_bot;  // LMAG one-sided z stencils on the bottom wall (npez > 1,
+                  // golden/fortran/3dmhdsub.f:1126, 1392, 1513 sit in an ELSE IF)
 };
 
 // Metric arrays on the device (global indexing, see r3d_hip.h).

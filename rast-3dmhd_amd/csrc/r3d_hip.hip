@@ -155,6 +155,8 @@ void fill_consts(r3d_hip_ctx* c) {
     C.lmag = f.lmag;
     C.izcon = f.izcon;
     C.bz_zero = f.lmag && f.ibcon == 0;
+    C.ohm = f.obeta * f.orm * f.ocv;
+    C.mag_bot = f.npez > 1;
     // top boundary branch (reference src/bcon.cpp:43-94)
     C.half_t = 0;
     if (f.tp != 0.0 && (f.itcon == 0 || f.itcon == 1)) {
@@ -229,13 +231,20 @@ int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     const StageArgs a = stage_args(c, stage, zmode, want_dt);
     const dim3 blk((TILE_Y + 3) * 64);
     int rc = timed(c, 0, [&] {
-#define R3D_STAGE(S, Z) \
-    hipLaunchKernelGGL((k_stage<TILE_Y, S, Z>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
-        if (stage == 0 && zmode == 0) R3D_STAGE(0, 0);
-        else if (stage == 0) R3D_STAGE(0, 1);
-        else if (stage == 1 && zmode == 0) R3D_STAGE(1, 0);
-        else if (stage == 1) R3D_STAGE(1, 2);
-        else R3D_STAGE(2, 2);
+#define R3D_STAGE(S, Z, MG) \
+    hipLaunchKernelGGL((k_stage<TILE_Y, S, Z, MG>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
+#define R3D_STAGES(MG)                                     \
+    if (stage == 0 && zmode == 0) R3D_STAGE(0, 0, MG);     \
+    else if (stage == 0) R3D_STAGE(0, 1, MG);              \
+    else if (stage == 1 && zmode == 0) R3D_STAGE(1, 0, MG); \
+    else if (stage == 1) R3D_STAGE(1, 2, MG);              \
+    else R3D_STAGE(2, 2, MG);
+        if (c->cfg.lmag) {
+            R3D_STAGES(true)
+        } else {
+            R3D_STAGES(false)
+        }
+#undef R3D_STAGES
 #undef R3D_STAGE
     });
     if (rc) return rc;
@@ -292,7 +301,6 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
         return set_err(R3D_EINVAL, "npey=%d / npez=%d must divide npy=%d / npz=%d", f.npey, f.npez,
                        f.npy, f.npz);
     if (f.npz / f.npez < 3) return set_err(R3D_EINVAL, "each virtual z rank needs >= 3 planes");
-    if (f.lmag) return set_err(R3D_EUNSUP, "FLUXES: unsupported feature enabled: LMAG");
     if (f.izcon != 0 && f.izcon != 1) return set_err(R3D_EINVAL, "BCON: invalid IZCON %d", f.izcon);
     if (f.itcon == 2 || f.itcon == 3)
         return set_err(R3D_EUNSUP, "FLUXES: unsupported feature enabled: ITCON embedded heat/thermal");

@@ -1,4 +1,4 @@
-// r3d_stage.hpp - the fused RK3 stage kernel for gfx950 (hydrodynamic path).
+// r3d_stage.hpp - the fused RK3 stage kernel for gfx950 (hydrodynamic and MHD).
 //
 // One kernel launch = one RK stage over the whole domain:
 //     F  = FLUXES(X)                      reference src/fluxes.cpp:44-533
@@ -15,6 +15,12 @@
 // its x/y neighbours need to a double-buffered LDS plane (one barrier per
 // plane).  Only waves 1..TY compute fluxes; the halo waves only publish.
 //
+// MHD (MAG = true) adds BX, BY, BZ to the state and the LMAG terms of the
+// Fortran FLUXES (golden/fortran/3dmhdsub.f:2146-2651): the Lorentz force on
+// FU, FV, FW, the Ohmic heating on FT and the induction + resistive diffusion
+// fluxes of B.  The reference C++ has no LMAG branch; these follow the
+// Fortran's left-to-right evaluation of each statement.
+//
 // Bit-exactness.  This file must be compiled with -ffp-contract=off: every
 // product and sum below is written in the operand order of the reference
 // CPU sweep it cites, so each intermediate rounds identically.  Quantities the
@@ -28,17 +34,18 @@
 namespace r3d {
 
 // Quantities a column publishes to its in-plane neighbours.
-enum { SH_RU = 0, SH_RV, SH_RW, SH_TT, SH_P, SH_U, SH_V, SH_W, SH_WZ, SH_UZ, SH_VZ, NSH };
+enum { SH_RU = 0, SH_RV, SH_RW, SH_TT, SH_P, SH_U, SH_V, SH_W, SH_WZ, SH_UZ, SH_VZ, NSH,
+       SH_BX = NSH, SH_BY, SH_BZ, NSH_MAG };
 
 struct Raw {
-    double ru, rv, rw, ro, tt;
+    double ru, rv, rw, ro, tt, bx, by, bz;
 };
 
 // Per-plane values of one column (reference src/fluxes.cpp:131-166: P = RO*TT
 // before the inversion, 1/RO, velocities = momentum * (1/RO)); ruw..rww are
 // the vertical advective fluxes m*w of src/fluxes.cpp:183-185.
 struct Col {
-    double ru, rv, rw, ro, tt, p, ri, u, v, w, ruw, rvw, rww;
+    double ru, rv, rw, ro, tt, p, ri, u, v, w, ruw, rvw, rww, bx, by, bz;
 };
 
 // Global accesses of the stage kernel use a plane base that is uniform over
@@ -55,6 +62,7 @@ __device__ __forceinline__ void stp(double* f, long pp, int k, uint32_t voff, do
     *reinterpret_cast<double*>(const_cast<char*>(plane_base(f, pp, k)) + voff) = v;
 }
 
+template <bool MAG>
 __device__ __forceinline__ Raw load_raw(const State& s, long pp, int k, uint32_t voff) {
     Raw r;
     r.ru = ldp(s.q[QRU], pp, k, voff);
@@ -62,6 +70,13 @@ __device__ __forceinline__ Raw load_raw(const State& s, long pp, int k, uint32_t
     r.rw = ldp(s.q[QRW], pp, k, voff);
     r.ro = ldp(s.q[QRO], pp, k, voff);
     r.tt = ldp(s.q[QTT], pp, k, voff);
+    if constexpr (MAG) {
+        r.bx = ldp(s.q[QBX], pp, k, voff);
+        r.by = ldp(s.q[QBY], pp, k, voff);
+        r.bz = ldp(s.q[QBZ], pp, k, voff);
+    } else {
+        r.bx = r.by = r.bz = 0.0;
+    }
     return r;
 }
 
@@ -80,6 +95,9 @@ __device__ __forceinline__ Col derive(const Raw& r) {
     c.ruw = c.ru * c.w;
     c.rvw = c.rv * c.w;
     c.rww = c.rw * c.w;
+    c.bx = r.bx;
+    c.by = r.by;
+    c.bz = r.bz;
     return c;
 }
 
@@ -105,17 +123,27 @@ __device__ __forceinline__ double wave_min(double v) {
 }
 
 // Per-cell time-step limits of a final state value (reference
-// src/step.cpp:125-209, hydrodynamic branch): pm[0] = max(ogamma*UU/T),
-// pm[1..3] = min of dd/sqrt(fast speed), the thermal and the viscous limit.
+// src/step.cpp:125-209): pm[0] = max(ogamma*UU/T), pm[1..3] = min of
+// dd/sqrt(fast speed), the thermal and the viscous limit, pm[4] = min of the
+// resistive limit (MHD).  With MAG the fast speed includes the Alfven term
+// b2 = (bx^2+by^2+bz^2)*obeta/ro (src/step.cpp:144-156).
+template <bool MAG>
 __device__ __forceinline__ void dt_terms(const Consts& C, double ru, double rv, double rw,
-                                         double ro, double tt, double dd, double rk,
-                                         double pm[NPART]) {
+                                         double ro, double tt, double bx, double by, double bz,
+                                         double dd, double rk, double pm[NPART]) {
     const double uu = (ru * ru + rv * rv + rw * rw) * (1.0 / ro) * (1.0 / ro);
     pm[0] = dmax(pm[0], C.og * uu / tt);
-    const double fast = uu + C.gamma * tt + 2.0 * sqrt(uu * C.gamma * tt);
+    double fast;
+    if constexpr (MAG) {
+        const double b2 = (bx * bx + by * by + bz * bz) * C.obeta / ro;
+        fast = uu + C.gamma * tt + b2 + 2.0 * sqrt(uu * (C.gamma * tt + b2));
+    } else {
+        fast = uu + C.gamma * tt + 2.0 * sqrt(uu * C.gamma * tt);
+    }
     pm[1] = dmin(pm[1], dd / sqrt(fast));
     pm[2] = dmin(pm[2], 0.5 * dd * dd * C.repr * C.cv * ro / rk);
     pm[3] = dmin(pm[3], 0.375 * dd * dd * C.re * ro);
+    if constexpr (MAG) pm[4] = dmin(pm[4], 0.5 * dd * dd * C.rm);
 }
 
 // Write the block's dt extrema to partial[slot*NPART ...].  red must hold
@@ -143,12 +171,13 @@ __device__ __forceinline__ void block_partials(double pm[NPART], double (*red)[N
 
 // ZMODE: 0 = Z' is the next stage's mix, 1 = Z keeps the saved state (a step
 // truncated after stage 1), 2 = Z is not written (last stage of the step).
-template <int TY, int STAGE, int ZMODE>
+template <int TY, int STAGE, int ZMODE, bool MAG>
 __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, const Consts C,
                                                          const Metrics M) {
     constexpr int ROWS = TY + 2;
     constexpr int NW = TY + 3;
-    __shared__ double sh[2][NSH][ROWS][66];
+    constexpr int NF = MAG ? 8 : 5;  // primaries per state
+    __shared__ double sh[2][MAG ? NSH_MAG : NSH][ROWS][66];
     __shared__ double red[NW][NPART];
     // Per-plane z metrics of this block's chunk, staged once: they are
     // uniform per plane, and fetching them inside the z loop would put a
@@ -213,10 +242,10 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
     const double dyy = M.dyy[lj], d2yy = M.d2yy[lj];
     const double ddxy = (STAGE == 2) ? dmin(M.ddx[li], M.ddy[lj]) : 0.0;
 
-    Col cm = derive(load_raw(a.xin, g.pp, k0 - 1, voff));
-    Col cc = derive(load_raw(a.xin, g.pp, k0, voff));
+    Col cm = derive(load_raw<MAG>(a.xin, g.pp, k0 - 1, voff));
+    Col cc = derive(load_raw<MAG>(a.xin, g.pp, k0, voff));
     double rw_m2 = ldp(a.xin.q[QRW], g.pp, max(k0 - 2, 0), voff);
-    Raw rn = load_raw(a.xin, g.pp, min(k0 + 1, g.npz + 1), voff);
+    Raw rn = load_raw<MAG>(a.xin, g.pp, min(k0 + 1, g.npz + 1), voff);
 
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
 
@@ -226,10 +255,10 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
         // access is then (uniform plane base) + zext(voff), the saddr form,
         // instead of one hoisted 64-bit per-lane address per field.
         asm volatile("" : "+v"(voff));
-        const Raw rnn = load_raw(a.xin, g.pp, min(k + 2, g.npz + 1), voff);
-        double zc[5];
+        const Raw rnn = load_raw<MAG>(a.xin, g.pp, min(k + 2, g.npz + 1), voff);
+        double zc[NF];
         if (STAGE > 0 && heavy)
-            for (int f = 0; f < 5; ++f) zc[f] = ldp(a.z.q[f], g.pp, k, voff);
+            for (int f = 0; f < NF; ++f) zc[f] = ldp(a.z.q[f], g.pp, k, voff);
         const Col cp = derive(rn);
         const int kl = k - k0;
         const double dzz = zm[0][kl], d2zz = zm[1][kl];
@@ -250,6 +279,11 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
             sh[buf][SH_WZ][r][c] = wz;
             sh[buf][SH_UZ][r][c] = uz;
             sh[buf][SH_VZ][r][c] = vz;
+            if constexpr (MAG) {
+                sh[buf][SH_BX][r][c] = cc.bx;
+                sh[buf][SH_BY][r][c] = cc.by;
+                sh[buf][SH_BZ][r][c] = cc.bz;
+            }
         }
         __syncthreads();
 
@@ -272,12 +306,17 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                 zc[2] = cc.rw;
                 zc[3] = cc.ro;
                 zc[4] = cc.tt;
+                if constexpr (MAG) {
+                    zc[5] = cc.bx;
+                    zc[6] = cc.by;
+                    zc[7] = cc.bz;
+                }
             }
             const double tmpz = C.hz * dzz;
             const double tmpz1 = C.hz * d2zz;
             const double tmpz2 = C.h2z * dzz * dzz;
             const double hxd = C.hx;
-            double xv[5];
+            double xv[NF];
             // RK update of component f (src/step.cpp:53-80), its stores and the
             // next stage's mix (src/step.cpp:83-110) or the saved state.
             // The stores are unconditional so that the five components form
@@ -296,11 +335,18 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                     // on the two BCON planes F is parked for the boundary
                     // kernel, which mixes after BCON (uniform select)
                     double* dst = bplane ? a.fb : a.z.q[f];
-                    const int kp = bplane ? (k == 1 ? 0 : 5) + f : k;
+                    const int kp = bplane ? (k == 1 ? 0 : NF) + f : k;
                     stp(dst, g.pp, kp, voff, bplane ? fval : x + cz * fval);
                 }
             };
 #define LAPX(e, ww, c2) ((e - ww) * hxd * d2xx + (e - c2 + ww) * C.h2x * dxx * dxx)
+// LMAG derivatives with the Fortran's left-to-right metric products
+// ((diff * h) * metric), golden/fortran/3dmhdsub.f:2150-2651
+#define DXS(q) ((SH(q, r, c + 1) - SH(q, r, c - 1)) * hxd * dxx)
+#define DYS(q) ((SH(q, r + 1, c) - SH(q, r - 1, c)) * C.hy * dyy)
+#define DZC(p_, m_) ((p_ - m_) * C.hz * dzz)
+#define LAPY(n_, s_, c2) ((n_ - s_) * C.hy * d2yy + (n_ - c2 + s_) * C.h2y * dyy * dyy)
+#define LAPZ(p_, m_, c2) ((p_ - m_) * C.hz * d2zz + (p_ - c2 + m_) * C.h2z * dzz * dzz)
             {  // ---- FR: continuity (src/fluxes.cpp:87-121) ----
                 const double w1top = (4.0 * cp.rw - rnn.rw) * tmpz;  // one-sided, top
                 const double w1bot = (rw_m2 - 4.0 * cm.rw) * tmpz;   // one-sided, bottom
@@ -330,6 +376,12 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                 fu = fu + C.t2 * (vxN - vxS) * C.hy * dyy;
                 fu = fu + C.t2 * (SH(SH_WZ, r, c + 1) - SH(SH_WZ, r, c - 1)) * hxd * dxx;
                 if (C.lrot) fu = fu + C.omz * cc.rv;
+                if constexpr (MAG) {  // Lorentz force, 3dmhdsub.f:1190, 1340
+                    const double bxy = DYS(SH_BX), byx = DXS(SH_BY), bzx = DXS(SH_BZ);
+                    const double bxz = DZC(cp.bx, cm.bx);
+                    fu = fu + C.obeta * cc.by * (bxy - byx) - C.obeta * cc.bz * bzx;
+                    fu = fu + C.obeta * cc.bz * bxz;
+                }
                 finish(QRU, fu, cc.ru);
             }
             
@@ -351,6 +403,12 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                 fv = fv + C.t2 * (uxN - uxS) * C.hy * dyy;
                 fv = fv + C.t2 * (SH(SH_WZ, r + 1, c) - SH(SH_WZ, r - 1, c)) * C.hy * dyy;
                 if (C.lrot) fv = fv + (-C.omz * cc.ru + C.omx * cc.rw);
+                if constexpr (MAG) {  // Lorentz force, 3dmhdsub.f:1200, 1460
+                    const double byx = DXS(SH_BY), bxy = DYS(SH_BX);
+                    const double byz = DZC(cp.by, cm.by), bzy = DYS(SH_BZ);
+                    fv = fv + C.obeta * cc.bx * (byx - bxy);
+                    fv = fv + C.obeta * cc.bz * (byz - bzy);
+                }
                 finish(QRV, fv, cc.rv);
             }
             
@@ -371,6 +429,13 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                 fw = fw + C.t2 * ((SH(SH_UZ, r, c + 1) - SH(SH_UZ, r, c - 1)) * hxd * dxx +
                                   (SH(SH_VZ, r + 1, c) - SH(SH_VZ, r - 1, c)) * C.hy * dyy);
                 if (C.lrot) fw = fw + -C.omx * cc.rv;
+                if constexpr (MAG) {  // Lorentz force, 3dmhdsub.f:1210, 1350, 1470
+                    const double bzx = DXS(SH_BZ), bxz = DZC(cp.bx, cm.bx);
+                    const double byz = DZC(cp.by, cm.by), bzy = DYS(SH_BZ);
+                    fw = fw + C.obeta * cc.bx * bzx;
+                    fw = fw - C.obeta * cc.bx * bxz;
+                    fw = fw - C.obeta * cc.by * (byz - bzy);
+                }
                 finish(QRW, fw, cc.rw);
             }
             
@@ -404,15 +469,116 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
                 const double wy = (SH(SH_W, r + 1, c) - SH(SH_W, r - 1, c)) * C.hy * dyy;
                 ft = ft + (uz * uz + vz * vz + wy * wy + 2.0 * vz * wy) * cc.ri * C.t1;
                 ft = ft + 2.0 * uz * wx * cc.ri * C.t1;
+                if constexpr (MAG) {  // Ohmic heating, 3dmhdsub.f:1220, 1360, 1480
+                    const double bxy = DYS(SH_BX), byx = DXS(SH_BY), bzx = DXS(SH_BZ);
+                    const double bxz = DZC(cp.bx, cm.bx);
+                    const double byz = DZC(cp.by, cm.by), bzy = DYS(SH_BZ);
+                    ft = ft + (bxy * bxy + byx * byx + bzx * bzx - 2.0 * byx * bxy) * cc.ri * C.ohm;
+                    ft = ft + (bxz * bxz - 2.0 * bxz * bzx) * cc.ri * C.ohm;
+                    ft = ft + (byz * byz + bzy * bzy - 2.0 * byz * bzy) * cc.ri * C.ohm;
+                }
                 finish(QTT, ft, cc.tt);
             }
+            if constexpr (MAG) {
+                // d(WW)/dz and the z second derivatives of BX, BY are one-sided
+                // on the walls (3dmhdsub.f:1125-1126, 1391-1392, 1511-1513); the
+                // bottom wall only when npez > 1 (ELSE IF).  k is uniform over
+                // the block, so these are scalar branches; the extra planes
+                // they need are read directly.
+                const bool wtop = k == 1, wbot = k == g.npz && C.mag_bot;
+                __builtin_amdgcn_sched_barrier(0);
+                {  // ---- WW1, flux of BX (3dmhdsub.f:1080, 1130, 1230, 1280, 1370, 1410) ----
+                    const double b2c = 2.0 * cc.bx;
+                    double wzm = DZC(cp.w, cm.w);
+                    double lbz = LAPZ(cp.bx, cm.bx, b2c);
+                    if (wtop) {
+                        const double b3 = ldp(a.xin.q[QBX], g.pp, 4, voff);
+                        wzm = (4.0 * cp.w - rnn.rw * (1.0 / rnn.ro)) * C.hz * dzz;
+                        lbz = (-3.0 * cc.bx + 4.0 * cp.bx - rnn.bx) * C.hz * d2zz +
+                              (2.0 * cc.bx - 5.0 * cp.bx + 4.0 * rnn.bx - b3) * C.h2z * dzz * dzz;
+                    } else if (wbot) {
+                        const double ro2 = ldp(a.xin.q[QRO], g.pp, k - 2, voff);
+                        const double b2 = ldp(a.xin.q[QBX], g.pp, k - 2, voff);
+                        const double b3 = ldp(a.xin.q[QBX], g.pp, k - 3, voff);
+                        wzm = (rw_m2 * (1.0 / ro2) - 4.0 * cm.w) * C.hz * dzz;
+                        lbz = (3.0 * cc.bx - 4.0 * cm.bx + b2) * C.hz * d2zz +
+                              (2.0 * cc.bx - 5.0 * cm.bx + 4.0 * b2 - b3) * C.h2z * dzz * dzz;
+                    }
+                    const double bE = SH(SH_BX, r, c + 1), bW = SH(SH_BX, r, c - 1);
+                    const double bN = SH(SH_BX, r + 1, c), bS = SH(SH_BX, r - 1, c);
+                    double w1 = cc.by * DYS(SH_U) - cc.bx * DYS(SH_V);
+                    w1 = w1 + cc.bz * DZC(cp.u, cm.u) - cc.bx * wzm;
+                    w1 = w1 - cc.u * DXS(SH_BX) - cc.v * DYS(SH_BX);
+                    w1 = w1 + C.orm * (LAPX(bE, bW, b2c) + LAPY(bN, bS, b2c));
+                    w1 = w1 - cc.w * DZC(cp.bx, cm.bx);
+                    w1 = w1 + C.orm * lbz;
+                    finish(QBX, w1, cc.bx);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                {  // ---- WW2, flux of BY (3dmhdsub.f:1030, 1140, 1240, 1290, 1380, 1420, 1490, 1530) ----
+                    const double b2c = 2.0 * cc.by;
+                    double wzm = DZC(cp.w, cm.w);
+                    double lbz = LAPZ(cp.by, cm.by, b2c);
+                    if (wtop) {
+                        const double b3 = ldp(a.xin.q[QBY], g.pp, 4, voff);
+                        wzm = (4.0 * cp.w - rnn.rw * (1.0 / rnn.ro)) * C.hz * dzz;
+                        lbz = (-3.0 * cc.by + 4.0 * cp.by - rnn.by) * C.hz * d2zz +
+                              (2.0 * cc.by - 5.0 * cp.by + 4.0 * rnn.by - b3) * C.h2z * dzz * dzz;
+                    } else if (wbot) {
+                        const double ro2 = ldp(a.xin.q[QRO], g.pp, k - 2, voff);
+                        const double b2 = ldp(a.xin.q[QBY], g.pp, k - 2, voff);
+                        const double b3 = ldp(a.xin.q[QBY], g.pp, k - 3, voff);
+                        wzm = (rw_m2 * (1.0 / ro2) - 4.0 * cm.w) * C.hz * dzz;
+                        lbz = (3.0 * cc.by - 4.0 * cm.by + b2) * C.hz * d2zz +
+                              (2.0 * cc.by - 5.0 * cm.by + 4.0 * b2 - b3) * C.h2z * dzz * dzz;
+                    }
+                    const double bE = SH(SH_BY, r, c + 1), bW = SH(SH_BY, r, c - 1);
+                    const double bN = SH(SH_BY, r + 1, c), bS = SH(SH_BY, r - 1, c);
+                    double w2 = cc.bx * DXS(SH_V) - cc.by * DXS(SH_U);
+                    w2 = w2 + cc.bz * DZC(cp.v, cm.v) - cc.by * wzm;
+                    w2 = w2 - cc.u * DXS(SH_BY);
+                    w2 = w2 + C.orm * LAPX(bE, bW, b2c);
+                    w2 = w2 - cc.v * DYS(SH_BY);
+                    w2 = w2 + C.orm * LAPY(bN, bS, b2c);
+                    w2 = w2 - cc.w * DZC(cp.by, cm.by);
+                    w2 = w2 + C.orm * lbz;
+                    finish(QBY, w2, cc.by);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                {  // ---- WW3, flux of BZ (3dmhdsub.f:1040, 1090, 1250, 1310, 1500-1580) ----
+                    const double b2c = 2.0 * cc.bz;
+                    const double bE = SH(SH_BZ, r, c + 1), bW = SH(SH_BZ, r, c - 1);
+                    const double bN = SH(SH_BZ, r + 1, c), bS = SH(SH_BZ, r - 1, c);
+                    double w3 = cc.bx * DXS(SH_W) - cc.bz * DXS(SH_U);
+                    w3 = w3 + cc.by * DYS(SH_W) - cc.bz * DYS(SH_V);
+                    w3 = w3 - cc.u * DXS(SH_BZ);
+                    w3 = w3 + C.orm * LAPX(bE, bW, b2c);
+                    w3 = w3 - cc.v * DYS(SH_BZ);
+                    w3 = w3 + C.orm * LAPY(bN, bS, b2c);
+                    w3 = w3 - cc.w * DZC(cp.bz, cm.bz);
+                    w3 = w3 + C.orm * LAPZ(cp.bz, cm.bz, b2c);
+                    finish(QBZ, w3, cc.bz);
+                }
+            }
 #undef LAPX
+#undef DXS
+#undef DYS
+#undef DZC
+#undef LAPY
+#undef LAPZ
 #undef SH
             if (STAGE == 2 && a.want_dt) {
                 // evaluated on every lane, folded in only on output cells
                 double q[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
                 const double dd = dmin(ddxy, zm[2][kl]);
-                dt_terms(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], dd, zm[3][kl], q);
+                double bxv = 0.0, byv = 0.0, bzv = 0.0;
+                if constexpr (MAG) {
+                    bxv = xv[QBX];
+                    byv = xv[QBY];
+                    bzv = xv[QBZ];
+                }
+                dt_terms<MAG>(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], bxv, byv, bzv, dd,
+                              zm[3][kl], q);
                 const bool use = outp && !bplane;
                 pm[0] = use ? dmax(pm[0], q[0]) : pm[0];
                 for (int v = 1; v < NPART; ++v) pm[v] = use ? dmin(pm[v], q[v]) : pm[v];

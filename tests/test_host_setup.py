@@ -33,14 +33,12 @@ def rank_view(glob, npey, npez, rank):
 
 @pytest.mark.parametrize("args", CASES, ids=[str(i) for i in range(len(CASES))])
 def test_initial_state_matches_oracle(args):
-    lmag = "--lmag" in args
-    if lmag:
-        pytest.skip("the oracle's LMAG branch is checked in test_mhd_*")
+    names = ("ru", "rv", "rw", "ro", "tt") + (("bx", "by", "bz") if "--lmag" in args else ())
     run = pkg.Run(args)
     sim = oracle.Sim(args)
     st = run.state()
     for r in range(sim.nranks):
-        for name in ("ru", "rv", "rw", "ro", "tt"):
+        for name in names:
             want = sim.field(r, name)
             got = rank_view(st[name], sim.npey, sim.npez, r)
             assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), (r, name)

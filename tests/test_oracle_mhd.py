@@ -158,10 +158,3 @@ def test_polynomial_fields_match_continuum_terms(seed):
             worst = max(worst, err)
             assert err < 1e-9, (r, k, err)
     assert worst < 1e-9
-
-
-def test_rank_needs_four_planes():
-    sim = oracle.Sim(["--npx", "8", "--npy", "8", "--npz", "12", "--npey", "1", "--npez", "4"]
-                     + MAG)
-    with pytest.raises(oracle.OracleError, match="4 interior planes"):
-        sim.fluxes(0)
