@@ -29,6 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "FP64 cell-updates/s per RK3 stage, % HBM roofline; adjoint:forward time ratio"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_STAGE = 400.0 / 3.0   # hydro: 10 passes x 5 fields x 8 B per step
+BYTES_PER_CELL_STAGE_MHD = 640.0 / 3.0  # MHD: 10 passes x 8 fields x 8 B per step
+MHD_FLAGS = ["--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2",
+             "--bzp", "10", "--lrot", "1", "--ry", "0.1", "--ang", "0"]
 CPU_CORES = 16                        # the GPU box's CPU share for one GPU
 
 
@@ -102,6 +105,10 @@ def main():
     ap.add_argument("--npz", type=int, default=2048)
     ap.add_argument("--npey", type=int, default=4)
     ap.add_argument("--npez", type=int, default=2)
+    ap.add_argument("--mhd", action="store_true",
+                    help="rotating MHD with the magnetic layer (Rm=50, beta=10, ampb=1, bfh=2, "
+                         "bzp=10, ry=0.1); use with --npx 252 --npy 252 --npz 512 --npey 2 "
+                         "--npez 4 for the MHD configuration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="before the run, time one device-to-device copy of this many GiB "
@@ -112,7 +119,10 @@ def main():
               "the multi-GPU sharding is a design, not a launched path", file=sys.stderr)
         sys.exit(2)
 
-    cpu = None if a.no_cpu_baseline else cpu_baseline(a.npx, a.npy, a.npey, a.npez, CPU_CORES)
+    extra = MHD_FLAGS if a.mhd else []
+    bpc = BYTES_PER_CELL_STAGE_MHD if a.mhd else BYTES_PER_CELL_STAGE
+    cpu = None if (a.no_cpu_baseline or a.mhd) else cpu_baseline(a.npx, a.npy, a.npey, a.npez,
+                                                                  CPU_CORES)
 
     if a.copy_calib_gib > 0:
         import torch
@@ -127,7 +137,7 @@ def main():
 
     pkg = load_package()
     args = ["--npx", str(a.npx), "--npy", str(a.npy), "--npz", str(a.npz),
-            "--npey", str(a.npey), "--npez", str(a.npez)]
+            "--npey", str(a.npey), "--npez", str(a.npez)] + extra
     log(f"[bench] host setup {' '.join(args)}")
     t0 = time.perf_counter()
     run = pkg.Run(args)
@@ -155,13 +165,14 @@ def main():
     cells = a.npx * a.npy * a.npz
     value = cells * 3 * a.steps / wall
     avg_stage_s = ms_stage / max(n_stage, 1) / 1e3
-    achieved = BYTES_PER_CELL_STAGE * cells / avg_stage_s / 1e9
+    achieved = bpc * cells / avg_stage_s / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")
     if os.path.exists(tfile):
         try:
             t = json.load(open(tfile))
-            if t.get("npx") == a.npx and t.get("npy") == a.npy and t.get("npz") == a.npz:
+            if (t.get("npx") == a.npx and t.get("npy") == a.npy and t.get("npz") == a.npz
+                    and bool(t.get("mhd", False)) == a.mhd):
                 traffic = t.get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
@@ -179,11 +190,12 @@ def main():
         "dtype": "f64",
         "data": "synthetic: the reference's static polytrope with its seeded ran2 "
                 "temperature perturbation (no dataset involved)",
-        "config": {"workload": f"hydro {a.npx}x{a.npy}x{a.npz}, virtual "
+        "config": {"workload": f"{'rotating MHD' if a.mhd else 'hydro'} "
+                               f"{a.npx}x{a.npy}x{a.npz}, virtual "
                                f"{a.npey}x{a.npez} subdomains on one GPU, bit-exact variant",
                    "global_cells": cells, "stages_per_step": 3,
                    "device_bytes": mem, "hbm_roofline_frac_of_value":
-                       value * BYTES_PER_CELL_STAGE / (HBM_PEAK_GBS * 1e9)},
+                       value * bpc / (HBM_PEAK_GBS * 1e9)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,

@@ -40,6 +40,12 @@ int set_err(int code, const char* fmt, ...) {
     } while (0)
 
 constexpr int TILE_Y = 8;  // rows per stage tile (waves 1..8 compute fluxes)
+#ifndef R3D_TILE_Y_MAG
+#define R3D_TILE_Y_MAG 5
+#endif
+// MHD tiles: 8 waves per workgroup (two per SIMD), so that the larger MHD
+// working set gets 256 VGPRs per lane instead of 168
+constexpr int TILE_Y_MAG = R3D_TILE_Y_MAG;
 constexpr int DT_BLOCKS = 1024;
 
 // Gather (dir = 1) or scatter (dir = 0) between the global array and one
@@ -87,7 +93,7 @@ struct r3d_hip_ctx {
     bool metrics_set = false;
     bool next_valid = false;
     std::vector<char> uploaded;
-    int ntx = 0, nty = 0, nkc = 0, kc = 0, nblocks = 0, grid = 0, nbb = 0;
+    int ty = TILE_Y, ntx = 0, nty = 0, nkc = 0, kc = 0, nblocks = 0, grid = 0, nbb = 0;
     unsigned long long bytes = 0;
     bool timing = false;
     long launches[4] = {0, 0, 0, 0};
@@ -229,20 +235,20 @@ StageArgs stage_args(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
 
 int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     const StageArgs a = stage_args(c, stage, zmode, want_dt);
-    const dim3 blk((TILE_Y + 3) * 64);
+    const dim3 blk((c->ty + 3) * 64);
     int rc = timed(c, 0, [&] {
-#define R3D_STAGE(S, Z, MG) \
-    hipLaunchKernelGGL((k_stage<TILE_Y, S, Z, MG>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
-#define R3D_STAGES(MG)                                     \
-    if (stage == 0 && zmode == 0) R3D_STAGE(0, 0, MG);     \
-    else if (stage == 0) R3D_STAGE(0, 1, MG);              \
-    else if (stage == 1 && zmode == 0) R3D_STAGE(1, 0, MG); \
-    else if (stage == 1) R3D_STAGE(1, 2, MG);              \
-    else R3D_STAGE(2, 2, MG);
+#define R3D_STAGE(TY, S, Z, MG) \
+    hipLaunchKernelGGL((k_stage<TY, S, Z, MG>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
+#define R3D_STAGES(TY, MG)                                      \
+    if (stage == 0 && zmode == 0) R3D_STAGE(TY, 0, 0, MG);      \
+    else if (stage == 0) R3D_STAGE(TY, 0, 1, MG);               \
+    else if (stage == 1 && zmode == 0) R3D_STAGE(TY, 1, 0, MG); \
+    else if (stage == 1) R3D_STAGE(TY, 1, 2, MG);               \
+    else R3D_STAGE(TY, 2, 2, MG);
         if (c->cfg.lmag) {
-            R3D_STAGES(true)
+            R3D_STAGES(TILE_Y_MAG, true)
         } else {
-            R3D_STAGES(false)
+            R3D_STAGES(TILE_Y, false)
         }
 #undef R3D_STAGES
 #undef R3D_STAGE
@@ -323,9 +329,10 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     c->uploaded.assign((size_t)f.npey * f.npez, 0);
     fill_consts(c);
 
-    // stage tiling: 64 x TILE_Y columns, z chunks sized for >= ~8 blocks per CU
+    // stage tiling: 64 x ty columns, z chunks sized for >= ~8 blocks per CU
+    c->ty = f.lmag ? TILE_Y_MAG : TILE_Y;
     c->ntx = (f.npx + 63) / 64;
-    c->nty = (f.npy + TILE_Y - 1) / TILE_Y;
+    c->nty = (f.npy + c->ty - 1) / c->ty;
     c->kc = 8;
     for (int kc : {128, 64, 32, 16, 8}) {
         if ((long)c->ntx * c->nty * ((f.npz + kc - 1) / kc) >= 2048) {
