@@ -169,12 +169,14 @@ def main():
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")
     if os.path.exists(tfile):
+        # HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
+        # WRITE_SIZE passes of this workload (null for any other workload)
         try:
-            t = json.load(open(tfile))
-            if (t.get("npx") == a.npx and t.get("npy") == a.npy and t.get("npz") == a.npz
-                    and bool(t.get("mhd", False)) == a.mhd):
-                traffic = t.get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
+            for t in json.load(open(tfile)):
+                if (t.get("npx") == a.npx and t.get("npy") == a.npy and t.get("npz") == a.npz
+                        and bool(t.get("mhd", False)) == a.mhd):
+                    traffic = t.get("hbm_bytes_per_launch")
+        except (OSError, ValueError, AttributeError):
             traffic = None
     out = {
         "metric": METRIC,
