@@ -8,7 +8,8 @@
 //             1/RO flip of the two outer ghost planes that the reference's
 //             FLUXES performs on its full array (src/fluxes.cpp:133-138) but
 //             its halo exchange never refreshes.
-// k_halo    : per stage, after k_bound: periodic x/y ghost images.
+// k_halo    : periodic x/y ghost images of a state, written before a download
+//             (the step itself reads wrapped interior cells instead).
 // k_dt      : dt limits of the current state when no fused value is
 //             available (first step, after an upload or a truncated step).
 // k_finish  : reduces the dt partials and advances the step counters.
@@ -143,9 +144,10 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt(const Geo g, const State s, c
     block_partials<DT_THREADS / 64>(pm, red, partial, blockIdx.x);
 }
 
-// Periodic x/y ghost images of X' on planes 1..npz (the reference's halo
-// exchange, src/communicate.cpp:70-169, for one global domain): every ghost
-// cell takes the interior cell it wraps to.  grid = (perimeter blocks, npz).
+// Periodic x/y ghost images on planes 1..npz (what the reference's halo
+// exchange, src/communicate.cpp:70-169, leaves in the ghost cells of one
+// global domain): every ghost cell takes the interior cell it wraps to.
+// grid = (perimeter blocks, npz).
 constexpr int HALO_THREADS = 256;
 __global__ __launch_bounds__(HALO_THREADS) void k_halo(const Geo g, const State x, int nf) {
     const int k = 1 + blockIdx.y;

@@ -259,12 +259,6 @@ int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
                            c->M, c->nblocks);
     });
     if (rc) return rc;
-    rc = timed(c, 3, [&] {
-        const int per = 2 * (c->cfg.npx + 2) + 2 * c->cfg.npy;
-        hipLaunchKernelGGL(k_halo, dim3((per + HALO_THREADS - 1) / HALO_THREADS, c->cfg.npz),
-                           dim3(HALO_THREADS), 0, c->stream, c->g, a.xout, c->nf);
-    });
-    if (rc) return rc;
     c->cur ^= 1;
     return R3D_OK;
 }
@@ -281,6 +275,17 @@ int launch_dt(r3d_hip_ctx* c) {
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, c->stream, c->scal, c->partial,
                            DT_BLOCKS, c->C, 1, 0);
     });
+}
+
+// Write the periodic ghost images of the current X (only downloads read them).
+int fill_ghosts(r3d_hip_ctx* c) {
+    State s;
+    for (int f = 0; f < NXF; ++f) s.q[f] = c->xs[c->cur][f];
+    const int per = 2 * (c->cfg.npx + 2) + 2 * c->cfg.npy;
+    hipLaunchKernelGGL(k_halo, dim3((per + HALO_THREADS - 1) / HALO_THREADS, c->cfg.npz),
+                       dim3(HALO_THREADS), 0, c->stream, c->g, s, c->nf);
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
 }
 
 int check_ready(r3d_hip_ctx* c) {
@@ -503,6 +508,8 @@ int r3d_hip_download_rank(r3d_hip_ctx* c, int rank, double* const* fields) {
     if (rank < 0 || rank >= f.npey * f.npez) return set_err(R3D_EINVAL, "rank %d out of range", rank);
     const int py = rank % f.npey, pz = rank / f.npey;
     const size_t n = (size_t)c->nx * c->ny * c->nz;
+    int rc = fill_ghosts(c);
+    if (rc) return rc;
     for (int q = 0; q < R3D_NFIELDS; ++q) {
         if (!fields[q]) continue;
         const bool isz = q >= R3D_ZRU;
@@ -527,6 +534,8 @@ int r3d_hip_download_global(r3d_hip_ctx* c, double* const* fields) {
     if (!c || !fields) return set_err(R3D_EINVAL, "null argument");
     const r3d_hip_cfg& f = c->cfg;
     const size_t W = (size_t)(f.npx + 2), rows = (size_t)(f.npy + 2) * (f.npz + 2);
+    int rc = fill_ghosts(c);
+    if (rc) return rc;
     for (int q = 0; q < R3D_NFIELDS; ++q) {
         if (!fields[q]) continue;
         const bool isz = q >= R3D_ZRU;

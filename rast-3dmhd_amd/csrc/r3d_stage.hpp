@@ -4,8 +4,15 @@
 //     F  = FLUXES(X)                      reference src/fluxes.cpp:44-533
 //     X' = Z + gam_s dt F                 reference src/step.cpp:53-80
 //     Z' = X' + zeta_s dt F               reference src/step.cpp:83-110 (next stage's mix)
-// plus the periodic x/y ghost images of X' (the halo exchange) and, in the
-// last stage, the per-cell dt limits of X' for the next step.
+// and, in the last stage, the per-cell dt limits of X' for the next step.
+//
+// Halo.  The reference refreshes ghost cells by a halo exchange after every
+// stage (src/communicate.cpp:70-169).  Here a virtual rank's ghost cells are
+// its neighbours' interior cells in the one global array, and the periodic
+// x/y ghosts are never read during a step: a column or row past the global
+// x/y edge is loaded from the interior cell it wraps to, which holds exactly
+// the value the halo exchange would have copied.  The ghost images in HBM are
+// written only when a state is downloaded (k_halo).
 //
 // Work decomposition.  A workgroup owns a 64 x TY column tile and a chunk of
 // kc planes.  Waves 0..TY+1 each own one row of 64 columns (rows j0-1..j0+TY,
@@ -221,9 +228,17 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
     }
     const bool heavy = (wv >= 1) && (wv <= TY);
     const bool outp = heavy && gi <= g.npx && gj <= g.npy;
-    const int li = min(gi, g.npx + 1), lj = min(gj, g.npy + 1);
+    // the column this lane reads: periodic wrap at the x/y edges; lanes past
+    // the wrap (partial edge tiles) read column/row 1 and their stores go to
+    // the row padding
+    const int li = gi == 0 ? g.npx : (gi > g.npx ? 1 : gi);
+    const int lj = gj == 0 ? g.npy : (gj > g.npy ? 1 : gj);
     // byte offset of this column within a plane (a plane is < 4 GiB)
     uint32_t voff = (uint32_t)((XOFF + li + (long)g.px * lj) * 8);
+    // Stores of a heavy lane that is not an output cell go to element 0 of
+    // its row, inside the unused row padding (interior i = 1 starts at
+    // element XOFF + 1), so that the store stream stays unconditional.
+    uint32_t soff = outp ? voff : (uint32_t)((long)g.px * lj * 8);
 
     for (int q = threadIdx.x; q < 5 * KCMAX; q += blockDim.x) {
         const int f = q / KCMAX, kk = k0 + q % KCMAX;
@@ -251,10 +266,11 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
 
     for (int k = k0; k <= k1; ++k) {
         const int buf = k & 1;
-        // Keep the lane offset opaque to loop-invariant code motion: every
+        // Keep the lane offsets opaque to loop-invariant code motion: every
         // access is then (uniform plane base) + zext(voff), the saddr form,
         // instead of one hoisted 64-bit per-lane address per field.
         asm volatile("" : "+v"(voff));
+        asm volatile("" : "+v"(soff));
         const Raw rnn = load_raw<MAG>(a.xin, g.pp, min(k + 2, g.npz + 1), voff);
         double zc[NF];
         if (STAGE > 0 && heavy)
@@ -319,24 +335,21 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
             double xv[NF];
             // RK update of component f (src/step.cpp:53-80), its stores and the
             // next stage's mix (src/step.cpp:83-110) or the saved state.
-            // The stores are unconditional so that the five components form
-            // one basic block the scheduler can interleave: a heavy lane that
-            // is not an output cell (past npx or npy in an edge tile) has its
-            // offset clamped to the ghost column npx+1 / ghost row npy+1, and
-            // whatever it writes there is replaced by k_halo's periodic
-            // images (X') or never read (Z, F on ghosts).
+            // The stores are unconditional so that the components form basic
+            // blocks the scheduler can interleave; a heavy lane that is not an
+            // output cell stores into its row's padding.
             auto finish = [&](int f, double fval, double cur) {
                 const double x = zc[f] + cg * fval;
                 xv[f] = x;
-                stp(a.xout.q[f], g.pp, k, voff, x);
+                stp(a.xout.q[f], g.pp, k, soff, x);
                 if (ZMODE == 1) {
-                    stp(a.z.q[f], g.pp, k, voff, cur);
+                    stp(a.z.q[f], g.pp, k, soff, cur);
                 } else if (ZMODE == 0 && STAGE < 2) {
                     // on the two BCON planes F is parked for the boundary
                     // kernel, which mixes after BCON (uniform select)
                     double* dst = bplane ? a.fb : a.z.q[f];
                     const int kp = bplane ? (k == 1 ? 0 : NF) + f : k;
-                    stp(dst, g.pp, kp, voff, bplane ? fval : x + cz * fval);
+                    stp(dst, g.pp, kp, soff, bplane ? fval : x + cz * fval);
                 }
             };
 #define LAPX(e, ww, c2) ((e - ww) * hxd * d2xx + (e - c2 + ww) * C.h2x * dxx * dxx)
