@@ -95,6 +95,58 @@ def load_package():
     return mod
 
 
+def run_ensemble(a):
+    """Aggregate cell-updates/s of the batched ensemble (autotuned tiles, graph
+    replay) next to the same members stepped one after another."""
+    pkg = load_package()
+    members = pkg.ensemble_members(a.ensemble)
+    runs = [pkg.Run(m) for m in members]
+    cells = sum(r.npx * r.npy * r.npz for r in runs)
+    log(f"[bench] ensemble of {len(runs)} members, {cells} cells")
+    ens = pkg.Ensemble(runs)
+    ms_tuned = ens.autotune(3)
+    ty, kc, nt = ens.tiles()
+    log(f"[bench] autotuned tiles: ty={ty} kc={kc} ({nt} tiles/stage), {ms_tuned:.3f} ms/step")
+    ens.step(a.warmup)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(a.steps)
+    ens.sync()
+    wall = time.perf_counter() - t0
+    mem = ens.memory_bytes()
+    ens.close()
+    value = cells * 3 * a.steps / wall
+    log(f"[bench] ensemble {wall / a.steps * 1e3:.3f} ms/step; sequential reference next")
+    seq = 0.0
+    for r in runs:
+        sim = r.make_sim()
+        sim.step(a.warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(a.steps)
+        sim.sync()
+        seq += time.perf_counter() - t0
+        sim.close()
+    seq_value = cells * 3 * a.steps / seq
+    achieved = BYTES_PER_CELL_STAGE * cells * 3 * a.steps / wall / 1e9
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded per-member polytropes and ran2 perturbations",
+        "config": {"workload": f"ensemble of {len(runs)} hydro members (48..96)^2 x 96..192, "
+                               "Re in [1,4], theta in [0.1,0.3], virtual 1x2 each",
+                   "global_cells": cells, "tiles": [ty, kc], "graph": True,
+                   "device_bytes": mem, "sequential_value": seq_value,
+                   "speedup_vs_sequential": value / seq_value},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": "whole-step rate (all kernels of the step), not one kernel"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +161,10 @@ def main():
                     help="rotating MHD with the magnetic layer (Rm=50, beta=10, ampb=1, bfh=2, "
                          "bzp=10, ry=0.1); use with --npx 252 --npy 252 --npz 512 --npey 2 "
                          "--npez 4 for the MHD configuration")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="M",
+                    help="batched ensemble of M seeded hydro members (sizes 48..96 x 96..192, "
+                         "Re in [1,4], theta in [0.1,0.3]) against the same members run one "
+                         "after another through the plain path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="before the run, time one device-to-device copy of this many GiB "
@@ -119,6 +175,8 @@ def main():
               "the multi-GPU sharding is a design, not a launched path", file=sys.stderr)
         sys.exit(2)
 
+    if a.ensemble:
+        return run_ensemble(a)
     extra = MHD_FLAGS if a.mhd else []
     bpc = BYTES_PER_CELL_STAGE_MHD if a.mhd else BYTES_PER_CELL_STAGE
     cpu = None if (a.no_cpu_baseline or a.mhd) else cpu_baseline(a.npx, a.npy, a.npey, a.npez,

@@ -87,6 +87,22 @@ def load():
         "r3d_hip_kernel_stats": [vp, ip, ctypes.POINTER(ctypes.c_long),
                                  ctypes.POINTER(ctypes.c_double)],
         "r3d_hip_memory": [vp, ctypes.POINTER(ctypes.c_ulonglong)],
+        # batched ensemble (include/r3d_ens.h)
+        "r3d_ens_create": [ip, ctypes.POINTER(Cfg), ctypes.POINTER(vp)],
+        "r3d_ens_destroy": [vp],
+        "r3d_ens_set_metrics": [vp, ip, ctypes.POINTER(Metrics)],
+        "r3d_ens_upload_global": [vp, ip, ctypes.POINTER(dp)],
+        "r3d_ens_download_global": [vp, ip, ctypes.POINTER(dp)],
+        "r3d_ens_download_rank": [vp, ip, ip, ctypes.POINTER(dp)],
+        "r3d_ens_set_scalars": [vp, ip, ctypes.POINTER(Scalars)],
+        "r3d_ens_get_scalars": [vp, ip, ctypes.POINTER(Scalars)],
+        "r3d_ens_set_tiles": [vp, ip, ip],
+        "r3d_ens_get_tiles": [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip)],
+        "r3d_ens_autotune": [vp, ip, ctypes.POINTER(ctypes.c_double)],
+        "r3d_ens_set_graph": [vp, ip],
+        "r3d_ens_step": [vp, ip],
+        "r3d_ens_sync": [vp],
+        "r3d_ens_memory": [vp, ctypes.POINTER(ctypes.c_ulonglong)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -208,6 +224,122 @@ class HipSim:
         b = ctypes.c_ulonglong()
         _check(load().r3d_hip_memory(self.h, ctypes.byref(b)))
         return b.value
+
+
+class Ensemble:
+    """Many independent runs advanced together on one GPU (include/r3d_ens.h).
+
+    runs: list of Run (host setups); each member gets that run's metrics,
+    initial state and zero scalars, exactly as Run.make_sim() does."""
+
+    def __init__(self, runs):
+        L = load()
+        cfgs = (Cfg * len(runs))(*[r.cfg() for r in runs])
+        h = ctypes.c_void_p()
+        _check(L.r3d_ens_create(len(runs), cfgs, ctypes.byref(h)))
+        self.h = h
+        self.n = len(runs)
+        self.cfgs = list(cfgs)
+        try:
+            for m, r in enumerate(runs):
+                md = r.metrics()
+                keep = {k: np.ascontiguousarray(md[k], dtype=np.float64) for k in METRIC_NAMES}
+                st = Metrics(*[_dptr(keep[k]) for k in METRIC_NAMES])
+                _check(L.r3d_ens_set_metrics(h, m, ctypes.byref(st)))
+                state = r.state(copy=False)
+                nf = 8 if self.cfgs[m].lmag else 5
+                arrs = [state[nm] for nm in PRIMARIES[:nf]]
+                ptrs = (ctypes.POINTER(ctypes.c_double) * 8)(*[_dptr(a) for a in arrs] +
+                                                            [None] * (8 - nf))
+                _check(L.r3d_ens_upload_global(h, m, ptrs))
+                _check(L.r3d_ens_set_scalars(h, m, ctypes.byref(Scalars())))
+        except Exception:
+            self.close()
+            raise
+
+    def close(self):
+        if self.h:
+            load().r3d_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1):
+        _check(load().r3d_ens_step(self.h, nsteps))
+
+    def sync(self):
+        _check(load().r3d_ens_sync(self.h))
+
+    def set_tiles(self, ty, kc=0):
+        _check(load().r3d_ens_set_tiles(self.h, ty, kc))
+
+    def tiles(self):
+        ty, kc, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(load().r3d_ens_get_tiles(self.h, ctypes.byref(ty), ctypes.byref(kc),
+                                        ctypes.byref(nt)))
+        return ty.value, kc.value, nt.value
+
+    def autotune(self, trial_steps=3):
+        ms = ctypes.c_double()
+        _check(load().r3d_ens_autotune(self.h, trial_steps, ctypes.byref(ms)))
+        return ms.value
+
+    def set_graph(self, on):
+        _check(load().r3d_ens_set_graph(self.h, 1 if on else 0))
+
+    def scalars(self, m):
+        s = Scalars()
+        _check(load().r3d_ens_get_scalars(self.h, m, ctypes.byref(s)))
+        return {"dt": s.dt, "timc": s.timc, "timt": s.timt, "timi": s.timi,
+                "umach": s.umach, "nit": s.nit}
+
+    def download_global(self, m, names=None):
+        names = names or FIELD_NAMES
+        c = self.cfgs[m]
+        out = {n: np.empty((c.npz + 2, c.npy + 2, c.npx + 2)) for n in names}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 16)()
+        for n in names:
+            ptrs[FIELD_NAMES.index(n)] = _dptr(out[n])
+        _check(load().r3d_ens_download_global(self.h, m, ptrs))
+        return out
+
+    def download_rank(self, m, rank, names=None):
+        names = names or FIELD_NAMES
+        c = self.cfgs[m]
+        shape = (c.npz // c.npez + 2, c.npy // c.npey + 2, c.npx + 2)
+        out = {n: np.empty(shape) for n in names}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 16)()
+        for n in names:
+            ptrs[FIELD_NAMES.index(n)] = _dptr(out[n])
+        _check(load().r3d_ens_download_rank(self.h, m, rank, ptrs))
+        return out
+
+    def memory_bytes(self):
+        b = ctypes.c_ulonglong()
+        _check(load().r3d_ens_memory(self.h, ctypes.byref(b)))
+        return b.value
+
+
+def ensemble_members(n=64, seed=3):
+    """Member flags of the seeded ensemble workload: horizontal size in
+    {48, 64, 80, 96} (square), vertical size in {96, 128, 160, 192},
+    Re uniform in [1, 4], theta uniform in [0.1, 0.3], ran2 seed -(1001 + m),
+    virtual 1 x 2 ranks, every other flag at the reference default."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n):
+        h = int(rng.choice([48, 64, 80, 96]))
+        v = int(rng.choice([96, 128, 160, 192]))
+        re = round(float(rng.uniform(1.0, 4.0)), 4)
+        th = round(float(rng.uniform(0.1, 0.3)), 4)
+        out.append(["--npx", str(h), "--npy", str(h), "--npz", str(v), "--npey", "1",
+                    "--npez", "2", "--re", repr(re), "--theta", repr(th),
+                    "--seed", str(-(1001 + m))])
+    return out
 
 
 # ----------------------------------------------------------------- host setup --

@@ -8,7 +8,7 @@
 //             1/RO flip of the two outer ghost planes that the reference's
 //             FLUXES performs on its full array (src/fluxes.cpp:133-138) but
 //             its halo exchange never refreshes.
-// k_halo    : periodic x/y ghost images of a state, written before a download
+// halo_thread: periodic x/y ghost images of a state, written before a download
 //             (the step itself reads wrapped interior cells instead).
 // k_dt      : dt limits of the current state when no fused value is
 //             available (first step, after an upload or a truncated step).
@@ -22,13 +22,14 @@ namespace r3d {
 constexpr int BOUND_THREADS = 256;
 constexpr int DT_THREADS = 256;
 
-__global__ __launch_bounds__(BOUND_THREADS) void k_bound(const StageArgs a, const Consts C,
-                                                         const Metrics M, int partial_base) {
+// Block bx of nbb (per side) of the boundary pass; side 0 = top, 1 = bottom.
+// Its dt partial goes to slot partial_base + side * nbb + bx.
+__device__ __forceinline__ void bound_block(const StageArgs& a, const Consts& C, const Metrics& M,
+                                            int partial_base, int bx, int nbb, int side) {
     __shared__ double red[BOUND_THREADS / 64][NPART];
     const Geo g = a.g;
-    const int side = blockIdx.y;  // 0 = top, 1 = bottom
     const int W = g.npx + 2, H = g.npy + 2;
-    const int idx = blockIdx.x * BOUND_THREADS + threadIdx.x;
+    const int idx = bx * BOUND_THREADS + threadIdx.x;
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
     const double dt = a.scal->dt;
     const double cz = (a.stage < 2) ? C.zeta[a.stage] * dt : 0.0;
@@ -115,19 +116,17 @@ __global__ __launch_bounds__(BOUND_THREADS) void k_bound(const StageArgs a, cons
         }
     }
     if (a.stage == 2 && a.want_dt)
-        block_partials<BOUND_THREADS / 64>(pm, red, a.partial,
-                                           partial_base + blockIdx.y * gridDim.x + blockIdx.x);
+        block_partials<BOUND_THREADS / 64>(pm, red, a.partial, partial_base + side * nbb + bx);
 }
 
-// dt limits of the state `s` (reference src/step.cpp:121-209), one partial
-// per block.
-__global__ __launch_bounds__(DT_THREADS) void k_dt(const Geo g, const State s, const Consts C,
-                                                   const Metrics M, double* partial) {
+// dt limits of the state `s` (reference src/step.cpp:121-209): block bx of
+// nbx, one partial per block.
+__device__ __forceinline__ void dt_block(const Geo& g, const State& s, const Consts& C,
+                                         const Metrics& M, double* partial, int bx, int nbx) {
     __shared__ double red[DT_THREADS / 64][NPART];
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
     const long total = (long)g.npx * g.npy * g.npz;
-    for (long q = (long)blockIdx.x * DT_THREADS + threadIdx.x; q < total;
-         q += (long)gridDim.x * DT_THREADS) {
+    for (long q = (long)bx * DT_THREADS + threadIdx.x; q < total; q += (long)nbx * DT_THREADS) {
         const int i = 1 + (int)(q % g.npx);
         const long rest = q / g.npx;
         const int j = 1 + (int)(rest % g.npy);
@@ -141,18 +140,17 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt(const Geo g, const State s, c
             dt_terms<false>(C, s.q[QRU][n], s.q[QRV][n], s.q[QRW][n], s.q[QRO][n], s.q[QTT][n],
                             0.0, 0.0, 0.0, dd, M.rkapa[k], pm);
     }
-    block_partials<DT_THREADS / 64>(pm, red, partial, blockIdx.x);
+    block_partials<DT_THREADS / 64>(pm, red, partial, bx);
 }
+
 
 // Periodic x/y ghost images on planes 1..npz (what the reference's halo
 // exchange, src/communicate.cpp:70-169, leaves in the ghost cells of one
 // global domain): every ghost cell takes the interior cell it wraps to.
 // grid = (perimeter blocks, npz).
 constexpr int HALO_THREADS = 256;
-__global__ __launch_bounds__(HALO_THREADS) void k_halo(const Geo g, const State x, int nf) {
-    const int k = 1 + blockIdx.y;
+__device__ __forceinline__ void halo_thread(const Geo& g, const State& x, int nf, int q, int k) {
     const int W = g.npx + 2;
-    const int q = blockIdx.x * HALO_THREADS + threadIdx.x;
     int i, j;
     if (q < 2 * W) {  // ghost rows j = 0 and j = npy+1, all i
         i = q % W;
@@ -174,8 +172,8 @@ __global__ __launch_bounds__(HALO_THREADS) void k_halo(const Geo g, const State 
 // umach_next (dt = sff * min of the limits, src/step.cpp:224-229; umach =
 // sqrt(max), src/step.cpp:141).  advance: the end-of-step counters
 // (src/step.cpp:254-256).
-__global__ __launch_bounds__(1024) void k_finish(DevScal* sc, const double* partial, int npart,
-                                                 const Consts C, int reduce, int advance) {
+__device__ __forceinline__ void finish_block(DevScal* sc, const double* partial, int npart,
+                                             const Consts& C, int reduce, int advance) {
     __shared__ double red[16][NPART];
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
     if (reduce)
@@ -210,5 +208,6 @@ __global__ __launch_bounds__(1024) void k_finish(DevScal* sc, const double* part
         }
     }
 }
+
 
 }  // namespace r3d

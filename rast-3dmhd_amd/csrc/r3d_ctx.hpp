@@ -1,0 +1,76 @@
+// r3d_ctx.hpp - host-side internals shared by the single-run library
+// (r3d_hip.hip) and the ensemble (r3d_ens.hip): the device context of one run
+// and the helpers that build and drive it.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "r3d_aux.hpp"
+#include "r3d_hip.h"
+
+namespace r3d {
+
+constexpr int TILE_Y = 8;  // rows per hydro stage tile (waves 1..8 compute fluxes)
+// MHD tiles: 8 waves per workgroup (two per SIMD), so that the larger MHD
+// working set gets 256 VGPRs per lane instead of 168
+constexpr int TILE_Y_MAG = 5;
+constexpr int DT_BLOCKS = 1024;
+
+// Set the calling thread's error message and return code.
+int set_err(int code, const char* fmt, ...);
+
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::r3d::set_err(R3D_EHIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace r3d
+
+// The device state of one run on one GPU.
+struct r3d_hip_ctx {
+    r3d_hip_cfg cfg;
+    r3d::Geo g;
+    int nf = 5;
+    int nx = 0, ny = 0, nz = 0, nry = 0, nrz = 0;
+    hipStream_t stream = nullptr;
+    double* xs[2][r3d::NXF] = {};  // X ping-pong; xs[cur] is the current state
+    double* z[r3d::NXF] = {};
+    int cur = 0;
+    double* fb = nullptr;
+    double* partial = nullptr;
+    r3d::DevScal* scal = nullptr;
+    double* metric_buf = nullptr;
+    double* stage_buf = nullptr;
+    r3d::Metrics M{};
+    r3d::Consts C{};
+    bool metrics_set = false;
+    bool next_valid = false;  // scal->dt_next holds the dt of the current state
+    std::vector<char> uploaded;
+    int ty = r3d::TILE_Y, ntx = 0, nty = 0, nkc = 0, kc = 0, nblocks = 0, grid = 0, nbb = 0;
+    unsigned long long bytes = 0;
+    bool timing = false;
+    long launches[4] = {0, 0, 0, 0};
+    double total_ms[4] = {0, 0, 0, 0};
+    std::vector<std::pair<int, hipEvent_t>> pending;  // (class, start event), end follows
+    std::vector<hipEvent_t> pending_end;
+};
+
+namespace r3d {
+
+// Elements of one padded global field (plus slack for the row padding).
+inline size_t field_elems(const Geo& g) { return (size_t)g.pp * (g.npz + 2) + 16; }
+
+// hipMalloc with the error mapped to R3D_ENOMEM; counts bytes on c.
+int dev_alloc(r3d_hip_ctx* c, double** p, size_t n);
+// Write the periodic ghost images of c's current X (downloads read them).
+int fill_ghosts(r3d_hip_ctx* c);
+// All virtual ranks and the metrics uploaded.
+int check_ready(r3d_hip_ctx* c);
+
+}  // namespace r3d

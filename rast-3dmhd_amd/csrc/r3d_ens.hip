@@ -1,0 +1,517 @@
+// r3d_ens.hip - batched ensemble of independent runs (include/r3d_ens.h).
+//
+// Every member is an ordinary run context (r3d_hip_ctx: its own buffers,
+// metrics, constants and scalars).  The ensemble adds a device table with one
+// descriptor per member and launches each RK stage once for all members: the
+// global tile index of a block is mapped to (member, tile) through the
+// members' first-tile offsets, so a stage of 64 small runs fills the GPU the
+// way one large run does.  The boundary pass, the dt reductions and the
+// end-of-step bookkeeping are likewise one launch each, indexed by member.
+// A step is captured in a HIP graph per X parity and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "r3d_ctx.hpp"
+#include "r3d_ens.h"
+
+using namespace r3d;
+
+namespace {
+
+// Device-side description of one member.
+struct EnsMember {
+    Geo g;
+    State x[2];   // X ping-pong
+    State z;
+    double* fb;
+    double* partial;
+    DevScal* scal;
+    int ntx, nty, nkc, kc, nblocks, nbb, tile0, npart_dt;
+    Consts C;
+    Metrics M;
+};
+
+constexpr int ENS_DT_BLOCKS = 64;  // blocks per member of the standalone dt pass
+
+__device__ __forceinline__ StageArgs member_args(const EnsMember& e, int cur, int stage,
+                                                 int zmode, int want_dt) {
+    StageArgs a;
+    a.g = e.g;
+    a.xin = e.x[cur];
+    a.xout = e.x[cur ^ 1];
+    a.z = e.z;
+    a.fb = e.fb;
+    a.partial = e.partial;
+    a.scal = e.scal;
+    a.stage = stage;
+    a.zmode = zmode;
+    a.want_dt = want_dt;
+    a.ntx = e.ntx;
+    a.nty = e.nty;
+    a.nkc = e.nkc;
+    a.kc = e.kc;
+    a.nblocks = e.nblocks;
+    return a;
+}
+
+// One stage of a full step for every member.  Tiles of one member are
+// contiguous in the global order, and the XCD-aware block order keeps each
+// member's tiles together on one XCD where it can.
+template <int TY, int STAGE, bool MAG>
+__global__ __launch_bounds__((TY + 3) * 64) void k_stage_ens(const EnsMember* __restrict__ mem,
+                                                             int nmem, int ntiles, int cur) {
+    const int T = logical_block();
+    if (T >= ntiles) return;
+    int lo = 0, hi = nmem - 1;  // last member whose first tile is <= T
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mem[mid].tile0 <= T) lo = mid;
+        else hi = mid - 1;
+    }
+    const EnsMember& e = mem[lo];
+    constexpr int ZMODE = STAGE < 2 ? 0 : 2;
+    const StageArgs a = member_args(e, cur, STAGE, ZMODE, STAGE == 2);
+    stage_tile<TY, STAGE, ZMODE, MAG>(a, e.C, e.M, T - e.tile0);
+}
+
+// grid = (max boundary blocks, 2 sides, members)
+__global__ __launch_bounds__(BOUND_THREADS) void k_bound_ens(const EnsMember* __restrict__ mem,
+                                                             int cur, int stage) {
+    const EnsMember& e = mem[blockIdx.z];
+    if ((int)blockIdx.x >= e.nbb) return;
+    const StageArgs a = member_args(e, cur, stage, stage < 2 ? 0 : 2, stage == 2);
+    bound_block(a, e.C, e.M, e.nblocks, blockIdx.x, e.nbb, blockIdx.y);
+}
+
+// grid = (ENS_DT_BLOCKS, members)
+__global__ __launch_bounds__(DT_THREADS) void k_dt_ens(const EnsMember* __restrict__ mem,
+                                                       int cur) {
+    const EnsMember& e = mem[blockIdx.y];
+    dt_block(e.g, e.x[cur], e.C, e.M, e.partial, blockIdx.x, gridDim.x);
+}
+
+// grid = (members); fused: partials of the last stage, else of k_dt_ens
+__global__ __launch_bounds__(1024) void k_finish_ens(const EnsMember* __restrict__ mem,
+                                                     int fused, int advance) {
+    const EnsMember& e = mem[blockIdx.x];
+    finish_block(e.scal, e.partial, fused ? e.nblocks + 2 * e.nbb : e.npart_dt, e.C, 1, advance);
+}
+
+struct Tiling {
+    int ty, kc;
+};
+
+}  // namespace
+
+struct r3d_ens_ctx {
+    std::vector<r3d_hip_ctx*> mem;
+    std::vector<double*> part;     // per-member partial buffers sized for any tiling
+    std::vector<EnsMember> desc;
+    EnsMember* d_desc = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0, lmag = 0;
+    Tiling tile{8, 0};
+    int ntiles = 0, grid = 0, nbb_max = 0;
+    int cur = 0;
+    bool use_graph = true;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    unsigned long long bytes = 0;
+};
+
+namespace {
+
+int member_kc(const r3d_ens_ctx* e, const r3d_hip_ctx* c) {
+    if (e->tile.kc > 0) return std::min(e->tile.kc, c->cfg.npz);
+    return std::min(32, c->cfg.npz);
+}
+
+void drop_graphs(r3d_ens_ctx* e) {
+    for (auto& gr : e->graph)
+        if (gr) {
+            (void)hipGraphExecDestroy(gr);
+            gr = nullptr;
+        }
+}
+
+// Rebuild the member table for the current tiling and parity.
+int build_table(r3d_ens_ctx* e) {
+    int tile0 = 0;
+    e->nbb_max = 0;
+    for (size_t m = 0; m < e->mem.size(); ++m) {
+        r3d_hip_ctx* c = e->mem[m];
+        EnsMember& d = e->desc[m];
+        d.g = c->g;
+        for (int s = 0; s < 2; ++s)
+            for (int f = 0; f < NXF; ++f) d.x[s].q[f] = c->xs[s][f];
+        for (int f = 0; f < NXF; ++f) d.z.q[f] = c->z[f];
+        d.fb = c->fb;
+        d.partial = e->part[m];
+        d.scal = c->scal;
+        d.ntx = (c->cfg.npx + 63) / 64;
+        d.nty = (c->cfg.npy + e->tile.ty - 1) / e->tile.ty;
+        d.kc = member_kc(e, c);
+        d.nkc = (c->cfg.npz + d.kc - 1) / d.kc;
+        d.nblocks = d.ntx * d.nty * d.nkc;
+        d.nbb = c->nbb;
+        d.tile0 = tile0;
+        d.npart_dt = ENS_DT_BLOCKS;
+        d.C = c->C;
+        d.M = c->M;
+        tile0 += d.nblocks;
+        e->nbb_max = std::max(e->nbb_max, d.nbb);
+    }
+    e->ntiles = tile0;
+    e->grid = (tile0 + 7) / 8 * 8;
+    HIPCHK(hipMemcpyAsync(e->d_desc, e->desc.data(), e->desc.size() * sizeof(EnsMember),
+                          hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    drop_graphs(e);
+    return R3D_OK;
+}
+
+template <int TY, bool MAG>
+void launch_stages_ty(r3d_ens_ctx* e, int stage, int cur) {
+    const dim3 blk((TY + 3) * 64);
+    const int n = (int)e->mem.size();
+    if (stage == 0)
+        hipLaunchKernelGGL((k_stage_ens<TY, 0, MAG>), dim3(e->grid), blk, 0, e->stream, e->d_desc,
+                           n, e->ntiles, cur);
+    else if (stage == 1)
+        hipLaunchKernelGGL((k_stage_ens<TY, 1, MAG>), dim3(e->grid), blk, 0, e->stream, e->d_desc,
+                           n, e->ntiles, cur);
+    else
+        hipLaunchKernelGGL((k_stage_ens<TY, 2, MAG>), dim3(e->grid), blk, 0, e->stream, e->d_desc,
+                           n, e->ntiles, cur);
+}
+
+// The launches of one full step starting from parity cur.
+int enqueue_step(r3d_ens_ctx* e, int cur) {
+    const int n = (int)e->mem.size();
+    for (int s = 0; s < 3; ++s) {
+        if (e->lmag) launch_stages_ty<TILE_Y_MAG, true>(e, s, cur);
+        else if (e->tile.ty == 4) launch_stages_ty<4, false>(e, s, cur);
+        else launch_stages_ty<8, false>(e, s, cur);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_bound_ens, dim3(e->nbb_max, 2, n), dim3(BOUND_THREADS), 0, e->stream,
+                           e->d_desc, cur, s);
+        HIPCHK(hipGetLastError());
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_finish_ens, dim3(n), dim3(1024), 0, e->stream, e->d_desc, 1, 1);
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
+}
+
+int enqueue_dt(r3d_ens_ctx* e) {
+    const int n = (int)e->mem.size();
+    hipLaunchKernelGGL(k_dt_ens, dim3(ENS_DT_BLOCKS, n), dim3(DT_THREADS), 0, e->stream,
+                       e->d_desc, e->cur);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_finish_ens, dim3(n), dim3(1024), 0, e->stream, e->d_desc, 0, 0);
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
+}
+
+int capture(r3d_ens_ctx* e, int cur) {
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step(e, cur);
+    const hipError_t ec = hipStreamEndCapture(e->stream, &gr);
+    if (rc) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+    }
+    if (ec != hipSuccess) return set_err(R3D_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&e->graph[cur], gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ei != hipSuccess) return set_err(R3D_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    return R3D_OK;
+}
+
+int check_member(r3d_ens_ctx* e, int m) {
+    if (!e) return set_err(R3D_EINVAL, "null ensemble");
+    if (m < 0 || m >= (int)e->mem.size()) return set_err(R3D_EINVAL, "member %d out of range", m);
+    return R3D_OK;
+}
+
+// Members' own bookkeeping follows the ensemble's parity.
+void sync_members(r3d_ens_ctx* e, bool next_valid) {
+    for (r3d_hip_ctx* c : e->mem) {
+        c->cur = e->cur;
+        c->next_valid = next_valid;
+    }
+}
+
+bool all_next_valid(const r3d_ens_ctx* e) {
+    for (const r3d_hip_ctx* c : e->mem)
+        if (!c->next_valid) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int r3d_ens_create(int nmembers, const r3d_hip_cfg* cfgs, r3d_ens_ctx** out) {
+    if (!cfgs || !out || nmembers < 1) return set_err(R3D_EINVAL, "bad arguments");
+    *out = nullptr;
+    for (int m = 0; m < nmembers; ++m) {
+        if (cfgs[m].device != cfgs[0].device || cfgs[m].lmag != cfgs[0].lmag)
+            return set_err(R3D_EINVAL, "member %d: all members need the same device and lmag", m);
+        if (cfgs[m].tc != 0.0)
+            return set_err(R3D_EUNSUP, "member %d: tc != 0 is not supported in an ensemble", m);
+    }
+    auto* e = new r3d_ens_ctx();
+    e->device = cfgs[0].device;
+    e->lmag = cfgs[0].lmag;
+    e->tile = {e->lmag ? TILE_Y_MAG : TILE_Y, 0};
+    auto fail = [&](int code) {
+        r3d_ens_destroy(e);
+        return code;
+    };
+    for (int m = 0; m < nmembers; ++m) {
+        r3d_hip_ctx* c = nullptr;
+        const int rc = r3d_hip_create(&cfgs[m], &c);
+        if (rc) {
+            const std::string msg = std::string("member ") + std::to_string(m) + ": " +
+                                    r3d_hip_last_error();
+            r3d_ens_destroy(e);
+            return set_err(rc, "%s", msg.c_str());
+        }
+        e->mem.push_back(c);
+        // partial slots for the finest tiling (4 rows, 8 planes) or the dt pass
+        const long worst = (long)((cfgs[m].npx + 63) / 64) * ((cfgs[m].npy + 3) / 4) *
+                               ((cfgs[m].npz + 7) / 8) + 2L * c->nbb;
+        double* p = nullptr;
+        const size_t np = (size_t)std::max<long>(worst, ENS_DT_BLOCKS) * NPART;
+        if (hipMalloc((void**)&p, np * sizeof(double)) != hipSuccess)
+            return fail(set_err(R3D_ENOMEM, "member %d: partial buffer", m));
+        e->part.push_back(p);
+        e->bytes += np * sizeof(double) + c->bytes;
+    }
+    e->desc.resize(nmembers);
+    if (hipMalloc((void**)&e->d_desc, nmembers * sizeof(EnsMember)) != hipSuccess)
+        return fail(set_err(R3D_ENOMEM, "member table"));
+    if (hipSetDevice(e->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(R3D_EHIP, "ensemble stream"));
+    *out = e;
+    return R3D_OK;
+}
+
+int r3d_ens_destroy(r3d_ens_ctx* e) {
+    if (!e) return R3D_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graphs(e);
+    for (double* p : e->part) (void)hipFree(p);
+    for (r3d_hip_ctx* c : e->mem) r3d_hip_destroy(c);
+    if (e->d_desc) (void)hipFree(e->d_desc);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return R3D_OK;
+}
+
+int r3d_ens_set_metrics(r3d_ens_ctx* e, int m, const r3d_hip_metrics* mt) {
+    int rc = check_member(e, m);
+    if (rc) return rc;
+    if ((rc = r3d_hip_set_metrics(e->mem[m], mt))) return rc;
+    e->ntiles = 0;  // constants and metric pointers go into the member table
+    return R3D_OK;
+}
+
+int r3d_ens_upload_global(r3d_ens_ctx* e, int m, const double* const* fields) {
+    int rc = check_member(e, m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    r3d_hip_ctx* c = e->mem[m];
+    // the upload writes both X buffers; the member follows the ensemble parity
+    if ((rc = r3d_hip_upload_global(c, fields))) return rc;
+    c->cur = e->cur;
+    return R3D_OK;
+}
+
+int r3d_ens_download_global(r3d_ens_ctx* e, int m, double* const* fields) {
+    int rc = check_member(e, m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return r3d_hip_download_global(e->mem[m], fields);
+}
+
+int r3d_ens_download_rank(r3d_ens_ctx* e, int m, int rank, double* const* fields) {
+    int rc = check_member(e, m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return r3d_hip_download_rank(e->mem[m], rank, fields);
+}
+
+int r3d_ens_set_scalars(r3d_ens_ctx* e, int m, const r3d_hip_scalars* s) {
+    int rc = check_member(e, m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return r3d_hip_set_scalars(e->mem[m], s);
+}
+
+int r3d_ens_get_scalars(r3d_ens_ctx* e, int m, r3d_hip_scalars* s) {
+    int rc = check_member(e, m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return r3d_hip_get_scalars(e->mem[m], s);
+}
+
+int r3d_ens_set_tiles(r3d_ens_ctx* e, int ty, int kc) {
+    if (!e) return set_err(R3D_EINVAL, "null ensemble");
+    const bool ok_ty = e->lmag ? ty == TILE_Y_MAG : (ty == 4 || ty == 8);
+    if (!ok_ty) return set_err(R3D_EINVAL, "tile rows %d not built for this ensemble", ty);
+    if (kc != 0 && (kc < 8 || kc > 128)) return set_err(R3D_EINVAL, "kc %d outside 8..128", kc);
+    e->tile = {ty, kc};
+    e->ntiles = 0;
+    return R3D_OK;
+}
+
+int r3d_ens_get_tiles(r3d_ens_ctx* e, int* ty, int* kc, int* ntiles) {
+    if (!e) return set_err(R3D_EINVAL, "null ensemble");
+    if (!e->ntiles) {
+        const int rc = build_table(e);
+        if (rc) return rc;
+    }
+    if (ty) *ty = e->tile.ty;
+    if (kc) *kc = e->tile.kc;
+    if (ntiles) *ntiles = e->ntiles;
+    return R3D_OK;
+}
+
+int r3d_ens_set_graph(r3d_ens_ctx* e, int enable) {
+    if (!e) return set_err(R3D_EINVAL, "null ensemble");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->use_graph = enable != 0;
+    return R3D_OK;
+}
+
+int r3d_ens_step(r3d_ens_ctx* e, int nsteps) {
+    if (!e) return set_err(R3D_EINVAL, "null ensemble");
+    if (nsteps < 0) return set_err(R3D_EINVAL, "nsteps < 0");
+    int rc;
+    for (r3d_hip_ctx* c : e->mem)
+        if ((rc = check_ready(c))) return rc;
+    if (!e->ntiles && (rc = build_table(e))) return rc;
+    if (nsteps == 0) return R3D_OK;
+    // members may have used their own streams (uploads, scalars)
+    for (r3d_hip_ctx* c : e->mem) HIPCHK(hipStreamSynchronize(c->stream));
+    if (!all_next_valid(e) && (rc = enqueue_dt(e))) return rc;
+    for (int st = 0; st < nsteps; ++st) {
+        if (e->use_graph) {
+            if (!e->graph[e->cur] && (rc = capture(e, e->cur))) return rc;
+            HIPCHK(hipGraphLaunch(e->graph[e->cur], e->stream));
+        } else if ((rc = enqueue_step(e, e->cur))) {
+            return rc;
+        }
+        e->cur ^= 1;  // three stages flip the X parity
+    }
+    sync_members(e, true);
+    return R3D_OK;
+}
+
+int r3d_ens_sync(r3d_ens_ctx* e) {
+    if (!e) return set_err(R3D_EINVAL, "null ensemble");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return R3D_OK;
+}
+
+int r3d_ens_autotune(r3d_ens_ctx* e, int trial_steps, double* best_ms) {
+    if (!e || trial_steps < 1) return set_err(R3D_EINVAL, "bad arguments");
+    int rc;
+    for (r3d_hip_ctx* c : e->mem)
+        if ((rc = check_ready(c))) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (r3d_hip_ctx* c : e->mem) HIPCHK(hipStreamSynchronize(c->stream));
+    // save every member's current X and scalars
+    std::vector<double*> save;
+    std::vector<DevScal> scal(e->mem.size());
+    auto release = [&] {
+        for (double* p : save) (void)hipFree(p);
+    };
+    const int cur0 = e->cur;
+    std::vector<char> valid0;
+    for (size_t m = 0; m < e->mem.size(); ++m) {
+        r3d_hip_ctx* c = e->mem[m];
+        valid0.push_back(c->next_valid);
+        const size_t fe = field_elems(c->g);
+        double* p = nullptr;
+        if (hipMalloc((void**)&p, fe * c->nf * sizeof(double)) != hipSuccess) {
+            release();
+            return set_err(R3D_ENOMEM, "autotune: state backup");
+        }
+        save.push_back(p);
+        for (int f = 0; f < c->nf; ++f)
+            HIPCHK(hipMemcpyAsync(p + f * fe, c->xs[cur0][f], fe * sizeof(double),
+                                  hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(&scal[m], c->scal, sizeof(DevScal), hipMemcpyDeviceToHost,
+                              e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    auto restore = [&]() -> int {
+        e->cur = cur0;
+        for (size_t m = 0; m < e->mem.size(); ++m) {
+            r3d_hip_ctx* c = e->mem[m];
+            const size_t fe = field_elems(c->g);
+            for (int f = 0; f < c->nf; ++f)
+                HIPCHK(hipMemcpyAsync(c->xs[cur0][f], save[m] + f * fe, fe * sizeof(double),
+                                      hipMemcpyDeviceToDevice, e->stream));
+            HIPCHK(hipMemcpyAsync(c->scal, &scal[m], sizeof(DevScal), hipMemcpyHostToDevice,
+                                  e->stream));
+            c->cur = cur0;
+            c->next_valid = valid0[m];
+        }
+        HIPCHK(hipStreamSynchronize(e->stream));
+        return R3D_OK;
+    };
+    std::vector<Tiling> cand;
+    if (e->lmag) {
+        for (int kc : {8, 16, 32, 64}) cand.push_back({TILE_Y_MAG, kc});
+    } else {
+        for (int ty : {4, 8})
+            for (int kc : {8, 16, 32, 64}) cand.push_back({ty, kc});
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    Tiling best = e->tile;
+    double best_t = 1e300;
+    rc = R3D_OK;
+    for (const Tiling& t : cand) {
+        e->tile = t;
+        if ((rc = build_table(e)) || (rc = r3d_ens_step(e, 1))) break;
+        if (hipEventRecord(t0, e->stream) != hipSuccess) { rc = set_err(R3D_EHIP, "event"); break; }
+        if ((rc = r3d_ens_step(e, trial_steps))) break;
+        if (hipEventRecord(t1, e->stream) != hipSuccess ||
+            hipEventSynchronize(t1) != hipSuccess) { rc = set_err(R3D_EHIP, "event"); break; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        if (ms < best_t) {
+            best_t = ms;
+            best = t;
+        }
+        if ((rc = restore())) break;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    const int rc2 = restore();
+    release();
+    if (rc) return rc;
+    if (rc2) return rc2;
+    e->tile = best;
+    if ((rc = build_table(e))) return rc;
+    if (best_ms) *best_ms = best_t / trial_steps;
+    return R3D_OK;
+}
+
+int r3d_ens_memory(r3d_ens_ctx* e, unsigned long long* bytes) {
+    if (!e || !bytes) return set_err(R3D_EINVAL, "null argument");
+    *bytes = e->bytes;
+    return R3D_OK;
+}
+
+}  // extern "C"

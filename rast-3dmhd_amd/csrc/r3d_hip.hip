@@ -13,16 +13,15 @@
 #include <string>
 #include <vector>
 
-#include "r3d_hip.h"
-#include "r3d_aux.hpp"
+#include "r3d_ctx.hpp"
 
 using namespace r3d;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int set_err(int code, const char* fmt, ...) {
+int r3d::set_err(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -32,21 +31,7 @@ int set_err(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIPCHK(call)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (call);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return set_err(R3D_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
-    } while (0)
-
-constexpr int TILE_Y = 8;  // rows per stage tile (waves 1..8 compute fluxes)
-#ifndef R3D_TILE_Y_MAG
-#define R3D_TILE_Y_MAG 5
-#endif
-// MHD tiles: 8 waves per workgroup (two per SIMD), so that the larger MHD
-// working set gets 256 VGPRs per lane instead of 168
-constexpr int TILE_Y_MAG = R3D_TILE_Y_MAG;
-constexpr int DT_BLOCKS = 1024;
+namespace {
 
 // Gather (dir = 1) or scatter (dir = 0) between the global array and one
 // virtual rank's local array (nx*ny*nz, x fastest).  Local (i, j, k) is global
@@ -72,39 +57,30 @@ __global__ void k_rank_copy(double* glob, double* loc, Geo g, int nx, int ny, in
     }
 }
 
+// Kernels of one run over the device functions of r3d_aux.hpp.
+__global__ __launch_bounds__(BOUND_THREADS) void k_bound(const StageArgs a, const Consts C,
+                                                         const Metrics M, int partial_base) {
+    bound_block(a, C, M, partial_base, blockIdx.x, gridDim.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(DT_THREADS) void k_dt(const Geo g, const State s, const Consts C,
+                                                   const Metrics M, double* partial) {
+    dt_block(g, s, C, M, partial, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(1024) void k_finish(DevScal* sc, const double* partial, int npart,
+                                                 const Consts C, int reduce, int advance) {
+    finish_block(sc, partial, npart, C, reduce, advance);
+}
+
+// grid = (perimeter blocks, npz): planes 1..npz
+__global__ __launch_bounds__(HALO_THREADS) void k_halo(const Geo g, const State x, int nf) {
+    halo_thread(g, x, nf, blockIdx.x * HALO_THREADS + threadIdx.x, 1 + blockIdx.y);
+}
+
 }  // namespace
 
-struct r3d_hip_ctx {
-    r3d_hip_cfg cfg;
-    Geo g;
-    int nf = 5;
-    int nx = 0, ny = 0, nz = 0, nry = 0, nrz = 0;
-    hipStream_t stream = nullptr;
-    double* xs[2][NXF] = {};
-    double* z[NXF] = {};
-    int cur = 0;
-    double* fb = nullptr;
-    double* partial = nullptr;
-    DevScal* scal = nullptr;
-    double* metric_buf = nullptr;
-    double* stage_buf = nullptr;
-    Metrics M{};
-    Consts C{};
-    bool metrics_set = false;
-    bool next_valid = false;
-    std::vector<char> uploaded;
-    int ty = TILE_Y, ntx = 0, nty = 0, nkc = 0, kc = 0, nblocks = 0, grid = 0, nbb = 0;
-    unsigned long long bytes = 0;
-    bool timing = false;
-    long launches[4] = {0, 0, 0, 0};
-    double total_ms[4] = {0, 0, 0, 0};
-    std::vector<std::pair<int, hipEvent_t>> pending;  // (class, start event), end follows
-    std::vector<hipEvent_t> pending_end;
-};
-
-namespace {
-
-int dev_alloc(r3d_hip_ctx* c, double** p, size_t n) {
+int r3d::dev_alloc(r3d_hip_ctx* c, double** p, size_t n) {
     hipError_t e = hipMalloc((void**)p, n * sizeof(double));
     if (e != hipSuccess) {
         *p = nullptr;
@@ -115,7 +91,7 @@ int dev_alloc(r3d_hip_ctx* c, double** p, size_t n) {
     return R3D_OK;
 }
 
-size_t field_elems(const Geo& g) { return (size_t)g.pp * (g.npz + 2) + 16; }
+namespace {
 
 void fill_consts(r3d_hip_ctx* c) {
     const r3d_hip_cfg& f = c->cfg;
@@ -277,8 +253,9 @@ int launch_dt(r3d_hip_ctx* c) {
     });
 }
 
-// Write the periodic ghost images of the current X (only downloads read them).
-int fill_ghosts(r3d_hip_ctx* c) {
+}  // namespace
+
+int r3d::fill_ghosts(r3d_hip_ctx* c) {
     State s;
     for (int f = 0; f < NXF; ++f) s.q[f] = c->xs[c->cur][f];
     const int per = 2 * (c->cfg.npx + 2) + 2 * c->cfg.npy;
@@ -288,14 +265,12 @@ int fill_ghosts(r3d_hip_ctx* c) {
     return R3D_OK;
 }
 
-int check_ready(r3d_hip_ctx* c) {
+int r3d::check_ready(r3d_hip_ctx* c) {
     if (!c->metrics_set) return set_err(R3D_ESTATE, "metrics not uploaded");
     for (size_t r = 0; r < c->uploaded.size(); ++r)
         if (!c->uploaded[r]) return set_err(R3D_ESTATE, "virtual rank %zu not uploaded", r);
     return R3D_OK;
 }
-
-}  // namespace
 
 extern "C" {
 

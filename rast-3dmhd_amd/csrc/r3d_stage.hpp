@@ -176,11 +176,12 @@ __device__ __forceinline__ void block_partials(double pm[NPART], double (*red)[N
     }
 }
 
-// ZMODE: 0 = Z' is the next stage's mix, 1 = Z keeps the saved state (a step
-// truncated after stage 1), 2 = Z is not written (last stage of the step).
+// One tile t (0 <= t < a.nblocks) of one RK stage; the caller maps blocks to
+// tiles.  ZMODE: 0 = Z' is the next stage's mix, 1 = Z keeps the saved state
+// (a step truncated after stage 1), 2 = Z is not written (last stage).
 template <int TY, int STAGE, int ZMODE, bool MAG>
-__global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, const Consts C,
-                                                         const Metrics M) {
+__device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, const Metrics& M,
+                                           const int t) {
     constexpr int ROWS = TY + 2;
     constexpr int NW = TY + 3;
     constexpr int NF = MAG ? 8 : 5;  // primaries per state
@@ -195,12 +196,10 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
     // Stage 1 takes the dt reduced at the end of the previous step (or by the
     // standalone dt pass) and makes it the step's dt.
     const double dt = (STAGE == 0) ? a.scal->dt_next : a.scal->dt;
-    if (STAGE == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (STAGE == 0 && t == 0 && threadIdx.x == 0) {
         a.scal->dt = a.scal->dt_next;
         a.scal->umach = a.scal->umach_next;
     }
-    const int t = logical_block();
-    if (t >= a.nblocks) return;
     const int ty = t % a.nty;
     const int rem = t / a.nty;
     const int tx = rem % a.ntx;
@@ -605,6 +604,15 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
     }
 
     if (STAGE == 2 && a.want_dt) block_partials<NW>(pm, red, a.partial, t);
+}
+
+// One RK stage of one run: block -> tile with the XCD-aware order.
+template <int TY, int STAGE, int ZMODE, bool MAG>
+__global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, const Consts C,
+                                                         const Metrics M) {
+    const int t = logical_block();
+    if (t >= a.nblocks) return;
+    stage_tile<TY, STAGE, ZMODE, MAG>(a, C, M, t);
 }
 
 }  // namespace r3d

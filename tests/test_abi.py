@@ -18,6 +18,7 @@ pkg = load_package()
 HEADERS = {
     "r3d_hip.h": os.path.join(ROOT, "rast-3dmhd_amd", "lib", "librast3dmhd_hip.so"),
     "r3d_host.h": os.path.join(ROOT, "rast-3dmhd_amd", "lib", "librast3dmhd_host.so"),
+    "r3d_ens.h": os.path.join(ROOT, "rast-3dmhd_amd", "lib", "librast3dmhd_hip.so"),
 }
 
 
@@ -58,6 +59,33 @@ def test_create_rejects_bad_config_without_a_device():
     # calls on a null context are errors, not crashes
     assert L.r3d_hip_step(None, 1, 3) == -1
     assert L.r3d_hip_destroy(None) == 0
+    assert L.r3d_ens_step(None, 1) == -1
+    assert L.r3d_ens_create(0, None, ctypes.byref(h)) == -1
+    assert L.r3d_ens_destroy(None) == 0
+
+
+def test_ensemble_rejects_mixed_members_without_a_device():
+    L = pkg.load()
+    cfgs = (pkg.Cfg * 2)()
+    cfgs[1].lmag = 1
+    h = ctypes.c_void_p()
+    assert L.r3d_ens_create(2, cfgs, ctypes.byref(h)) == -1
+    assert b"same device and lmag" in L.r3d_hip_last_error()
+    cfgs[1].lmag = 0
+    cfgs[0].tc = 1.0
+    assert L.r3d_ens_create(2, cfgs, ctypes.byref(h)) == -5
+
+
+def test_ensemble_members_are_seeded_and_in_range():
+    a = pkg.ensemble_members(64)
+    assert a == pkg.ensemble_members(64)
+    assert len({tuple(x) for x in a}) == 64
+    for m, args in enumerate(a):
+        d = dict(zip(args[::2], args[1::2]))
+        assert d["--npx"] == d["--npy"] and int(d["--npx"]) in (48, 64, 80, 96)
+        assert int(d["--npz"]) in (96, 128, 160, 192)
+        assert 1.0 <= float(d["--re"]) <= 4.0 and 0.1 <= float(d["--theta"]) <= 0.3
+        assert d["--seed"] == str(-(1001 + m))
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no host C++ compiler")

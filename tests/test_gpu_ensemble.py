@@ -1,0 +1,136 @@
+"""Batched ensemble (include/r3d_ens.h): every member bit-identical to the same
+run stepped alone through the plain path, whatever the tile shape, with or
+without graph replay, and after the autotuner's trial runs (which must leave
+the members' state untouched).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import load_package
+
+pkg = load_package()
+pytestmark = pytest.mark.gpu
+
+X_FIELDS = ["ru", "rv", "rw", "ro", "tt"]
+XZ_FIELDS = X_FIELDS + ["zru", "zrv", "zrw", "zro", "ztt"]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def single_run(args, nsteps):
+    sim = pkg.Run(args).make_sim()
+    sim.step(nsteps)
+    out = sim.download_global(XZ_FIELDS)
+    sc = sim.scalars()
+    sim.close()
+    return out, sc
+
+
+def compare_member(ens, m, want, want_sc, tag):
+    got = ens.download_global(m, XZ_FIELDS)
+    bad = []
+    for n in XZ_FIELDS:
+        g, w = got[n], want[n]
+        if n.startswith("z"):
+            g, w = g[1:-1, 1:-1, 1:-1], w[1:-1, 1:-1, 1:-1]
+        if not np.array_equal(g.view(np.uint64), w.view(np.uint64)):
+            bad.append((n, int((g != w).sum())))
+    sc = ens.scalars(m)
+    for k in ("dt", "timc", "timt", "umach", "nit"):
+        if sc[k] != want_sc[k]:
+            bad.append((k, sc[k], want_sc[k]))
+    assert not bad, f"{tag} member {m}: {bad[:6]}"
+
+
+SMALL = [
+    ["--npx", "48", "--npy", "48", "--npz", "96", "--npey", "1", "--npez", "2", "--re", "1.5",
+     "--theta", "0.12", "--seed", "-5"],
+    ["--npx", "80", "--npy", "80", "--npz", "64", "--npey", "2", "--npez", "2", "--re", "3.25",
+     "--theta", "0.27", "--seed", "-6"],
+    ["--npx", "20", "--npy", "12", "--npz", "30", "--npey", "1", "--npez", "3", "--itcon", "0",
+     "--izcon", "1", "--seed", "-7"],
+    ["--npx", "64", "--npy", "40", "--npz", "48", "--npey", "2", "--npez", "2", "--lrot", "1",
+     "--ry", "0.2", "--ang", "0.3", "--seed", "-8"],
+]
+
+
+@pytest.mark.parametrize("tiles", [(8, 0), (4, 16), (8, 64), (4, 8)],
+                         ids=["ty8_default", "ty4_kc16", "ty8_kc64", "ty4_kc8"])
+def test_ensemble_members_match_single_runs(tiles):
+    say(f"[gpu] ensemble of {len(SMALL)} uneven members, tiles {tiles}")
+    want = [single_run(a, 6) for a in SMALL]
+    ens = pkg.Ensemble([pkg.Run(a) for a in SMALL])
+    ens.set_tiles(*tiles)
+    ens.step(2)
+    ens.set_graph(False)  # plain launches in the middle
+    ens.step(1)
+    ens.set_graph(True)
+    ens.step(3)
+    for m in range(len(SMALL)):
+        compare_member(ens, m, *want[m], f"tiles {tiles}")
+    ens.close()
+
+
+def test_autotune_restores_state_and_keeps_results():
+    say("[gpu] ensemble autotune")
+    want = [single_run(a, 4) for a in SMALL]
+    ens = pkg.Ensemble([pkg.Run(a) for a in SMALL])
+    ens.step(1)
+    ms = ens.autotune(2)
+    ty, kc, nt = ens.tiles()
+    say(f"[gpu]   chose ty={ty} kc={kc} ({nt} tiles), {ms:.3f} ms/step")
+    assert ty in (4, 8) and kc in (8, 16, 32, 64) and ms > 0
+    ens.step(3)
+    for m in range(len(SMALL)):
+        compare_member(ens, m, *want[m], "after autotune")
+    ens.close()
+
+
+def test_ensemble_mhd_members_match_single_runs():
+    mag = ["--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2",
+           "--bzp", "10", "--lrot", "1", "--ry", "0.1", "--ang", "0"]
+    members = [
+        ["--npx", "32", "--npy", "32", "--npz", "48", "--npey", "2", "--npez", "2", "--seed",
+         "-11"] + mag,
+        ["--npx", "70", "--npy", "24", "--npz", "40", "--npey", "1", "--npez", "2", "--seed",
+         "-12", "--re", "3.5"] + mag,
+    ]
+    say("[gpu] MHD ensemble")
+    want = [single_run(a, 3) for a in members]
+    xs = ["bx", "by", "bz"]
+    want_b = []
+    for a in members:
+        sim = pkg.Run(a).make_sim()
+        sim.step(3)
+        want_b.append(sim.download_global(xs))
+        sim.close()
+    ens = pkg.Ensemble([pkg.Run(a) for a in members])
+    ens.step(3)
+    for m in range(len(members)):
+        compare_member(ens, m, *want[m], "mhd")
+        got = ens.download_global(m, xs)
+        for n in xs:
+            assert np.array_equal(got[n], want_b[m][n]), (m, n)
+    ens.close()
+
+
+def test_ensemble_workload_64_members_200_steps():
+    """The 64-member workload (sizes 48..96 x 96..192, Re, theta, seeds drawn
+    by ensemble_members), 200 steps: every member bit-identical to its own
+    plain run."""
+    members = pkg.ensemble_members(64)
+    say("[gpu] 64-member ensemble, 200 steps")
+    ens = pkg.Ensemble([pkg.Run(a) for a in members])
+    ens.autotune(2)
+    ens.step(200)
+    ens.sync()
+    for m, a in enumerate(members):
+        if m % 16 == 0:
+            say(f"[gpu]   member {m}")
+        compare_member(ens, m, *single_run(a, 200), "64-member workload")
+    ens.close()
