@@ -1,0 +1,119 @@
+// adj_check.hip - CPU harness for the per-cell flux and its adjoint
+// (rast-3dmhd_amd/csrc/r3d_adj.hpp).  TEST INFRASTRUCTURE: compiled for the
+// host only by tests/test_adjoint_cell.py; it runs cell_flux and
+// cell_flux_vjp over one rank array (x fastest, one ghost layer).
+#include <cmath>
+
+#include "r3d_adj.hpp"
+
+using namespace r3d;
+
+namespace {
+
+struct Dims {
+    int nx, ny, nz, top, bot;  // top/bot: the rank holds the global top/bottom wall
+};
+
+long at(const Dims& d, int i, int j, int k) { return ((long)k * d.ny + j) * d.nx + i; }
+
+void gather(const Dims& d, const double* const* X, int i, int j, int k, Stencil& s,
+            CellMet& m, const double* const* met) {
+    for (int q = 0; q < NPTS; ++q) {
+        int di, dj, dk;
+        pt_offset(q, di, dj, dk);
+        const long n = at(d, i + di, j + dj, k + dk);
+        s.p[q] = {X[0][n], X[1][n], X[2][n], X[3][n], X[4][n]};
+    }
+    m.wall = (d.top && k == 1) ? 1 : ((d.bot && k == d.nz - 2) ? 2 : 0);
+    s.rw2 = m.wall == 1 ? X[2][at(d, i, j, k + 2)] : (m.wall == 2 ? X[2][at(d, i, j, k - 2)] : 0.0);
+    m.dxx = met[0][i];
+    m.d2xx = met[1][i];
+    m.dyy = met[2][j];
+    m.d2yy = met[3][j];
+    m.dzz = met[4][k];
+    m.d2zz = met[5][k];
+    m.rk = met[6][k];
+    m.dk = met[7][k];
+}
+
+}  // namespace
+
+extern "C" {
+
+// consts: c13 c43 ore ocv grav hx h2x hy h2y hz h2z omx omz tk t1 t2 t3 lrot
+static Consts make_consts(const double* k) {
+    Consts C{};
+    C.c13 = k[0];
+    C.c43 = k[1];
+    C.ore = k[2];
+    C.ocv = k[3];
+    C.grav = k[4];
+    C.hx = k[5];
+    C.h2x = k[6];
+    C.hy = k[7];
+    C.h2y = k[8];
+    C.hz = k[9];
+    C.h2z = k[10];
+    C.omx = k[11];
+    C.omz = k[12];
+    C.tk = k[13];
+    C.t1 = k[14];
+    C.t2 = k[15];
+    C.t3 = k[16];
+    C.lrot = (int)k[17];
+    return C;
+}
+
+// F[f] over the interior (ghost entries untouched); X = ru rv rw ro tt;
+// met = dxx d2xx dyy d2yy dzz d2zz rkapa dkapa; F = fu fv fw fr ft.
+void adj_flux(const int* dims, const double* const* X, const double* const* met,
+              const double* kc, double* const* F) {
+    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4]};
+    const Consts C = make_consts(kc);
+    for (int k = 1; k <= d.nz - 2; ++k)
+        for (int j = 1; j <= d.ny - 2; ++j)
+            for (int i = 1; i <= d.nx - 2; ++i) {
+                Stencil s;
+                CellMet m;
+                gather(d, X, i, j, k, s, m, met);
+                double f[5];
+                cell_flux(s, m, C, f);
+                const long n = at(d, i, j, k);
+                for (int q = 0; q < 5; ++q) F[q][n] = f[q];
+            }
+}
+
+// lamX (whole array, ghosts included, accumulated) = (dF/dX)^T lamF over the
+// interior cells; g[0] = d(lamF.F)/d(ore), g[1] = d/d(tk).
+void adj_vjp(const int* dims, const double* const* X, const double* const* met, const double* kc,
+             const double* const* lamF, double* const* lamX, double* g) {
+    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4]};
+    const Consts C = make_consts(kc);
+    g[0] = g[1] = 0.0;
+    for (int k = 1; k <= d.nz - 2; ++k)
+        for (int j = 1; j <= d.ny - 2; ++j)
+            for (int i = 1; i <= d.nx - 2; ++i) {
+                Stencil s, A;
+                CellMet m;
+                gather(d, X, i, j, k, s, m, met);
+                const long n = at(d, i, j, k);
+                const double lam[5] = {lamF[0][n], lamF[1][n], lamF[2][n], lamF[3][n], lamF[4][n]};
+                double go, gt;
+                cell_flux_vjp(s, m, C, lam, A, go, gt);
+                g[0] += go;
+                g[1] += gt;
+                for (int q = 0; q < NPTS; ++q) {
+                    int di, dj, dk;
+                    pt_offset(q, di, dj, dk);
+                    const long p = at(d, i + di, j + dj, k + dk);
+                    lamX[0][p] += A.p[q].ru;
+                    lamX[1][p] += A.p[q].rv;
+                    lamX[2][p] += A.p[q].rw;
+                    lamX[3][p] += A.p[q].ro;
+                    lamX[4][p] += A.p[q].tt;
+                }
+                if (m.wall) lamX[2][at(d, i, j, m.wall == 1 ? k + 2 : k - 2)] += A.rw2;
+            }
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+"""The per-cell flux and its hand-written adjoint (rast-3dmhd_amd/csrc/r3d_adj.hpp),
+run on the CPU through tests/adjcheck/adj_check.hip:
+
+* cell_flux over a rank array reproduces the oracle's FLUXES (pinned to the
+  reference CPU build) bit for bit, walls and rotation included;
+* cell_flux_vjp is the transpose of the flux Jacobian: for random directions
+  the adjoint's inner product matches a central finite difference of
+  lam . F, field by field, and likewise for the coefficients ore and tk.
+"""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT
+
+ARGS = ["--npx", "12", "--npy", "10", "--npz", "16", "--npey", "1", "--npez", "2",
+        "--lrot", "1", "--ry", "0.3", "--ang", "0.4", "--re", "1.7", "--theta", "0.21"]
+GRAV = 0.625
+X_NAMES = ["ru", "rv", "rw", "ro", "tt"]
+F_NAMES = ["fu", "fv", "fw", "fr", "ft"]
+MET = ["dzzdz", "d2zzdz2", "rkapa", "dkapa"]
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    hipcc = "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    out = tmp_path_factory.mktemp("adj") / "libadjcheck.so"
+    subprocess.run([hipcc, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-fPIC",
+                    "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(ROOT, "rast-3dmhd_amd", "csrc"), "-shared", "-o", str(out),
+                    os.path.join(ROOT, "tests", "adjcheck", "adj_check.hip")],
+                   check=True, capture_output=True)
+    L = ctypes.CDLL(str(out))
+    pp = ctypes.POINTER(ctypes.POINTER(ctypes.c_double))
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.adj_flux.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, dp, pp]
+    L.adj_vjp.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, dp, pp, pp, dp]
+    return L
+
+
+def _pp(arrs):
+    return (ctypes.POINTER(ctypes.c_double) * len(arrs))(
+        *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in arrs])
+
+
+class Rank:
+    def __init__(self, sim, r):
+        self.sim, self.r = sim, r
+        py, pz = sim.rank_coords(r)
+        self.dims = (ctypes.c_int * 5)(sim.nx, sim.ny, sim.nz, int(pz == 0),
+                                       int(pz == sim.npez - 1))
+        self.met = [np.ascontiguousarray(sim.metric(r, n)) for n in
+                    ("dxxdx", "d2xxdx2", "dyydy", "d2yydy2", "dzzdz", "d2zzdz2", "rkapa", "dkapa")]
+        d = sim.derived()
+        self.d = d
+        self.k = self.consts(d["ore"], d["ocv"] / d["repr"])
+
+    def consts(self, ore, tk):
+        d = self.d
+        return np.array([d["c13"], d["c43"], ore, d["ocv"], GRAV, d["hx"], d["h2x"], d["hy"],
+                         d["h2y"], d["hz"], d["h2z"], d["omx"], d["omz"], tk, d["ocv"] * ore,
+                         d["c13"] * ore, d["c43"] * d["ocv"] * ore, 1.0])
+
+    def flux(self, L, X, k=None):
+        F = [np.zeros_like(X[0]) for _ in range(5)]
+        kk = self.k if k is None else k
+        L.adj_flux(self.dims, _pp(X), _pp(self.met), kk.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)), _pp(F))
+        return F
+
+    def vjp(self, L, X, lam):
+        out = [np.zeros_like(X[0]) for _ in range(5)]
+        g = np.zeros(2)
+        L.adj_vjp(self.dims, _pp(X), _pp(self.met),
+                  self.k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _pp(lam), _pp(out),
+                  g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out, g
+
+
+@pytest.fixture(scope="module")
+def sim():
+    s = oracle.Sim(ARGS)
+    s.step(3)
+    return s
+
+
+def test_cell_flux_reproduces_oracle_fluxes(lib, sim):
+    for r in range(sim.nranks):
+        rk = Rank(sim, r)
+        X = [sim.field(r, n).copy() for n in X_NAMES]
+        F = rk.flux(lib, X)
+        sim.fluxes(r)
+        for f, name in zip(F, F_NAMES):
+            want = sim.field(r, name)[1:-1, 1:-1, 1:-1]
+            got = f[1:-1, 1:-1, 1:-1]
+            assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), (r, name)
+        for q, n in enumerate(X_NAMES):  # undo the oracle's in-place 1/RO
+            sim.field(r, n)[...] = X[q]
+
+
+def _interior_mask(shape):
+    m = np.zeros(shape)
+    m[1:-1, 1:-1, 1:-1] = 1.0
+    return m
+
+
+@pytest.mark.parametrize("field", range(5), ids=X_NAMES)
+def test_vjp_matches_finite_differences(lib, sim, field):
+    rng = np.random.default_rng(17 + field)
+    for r in range(sim.nranks):
+        rk = Rank(sim, r)
+        X = [sim.field(r, n).copy() for n in X_NAMES]
+        mask = _interior_mask(X[0].shape)
+        lam = [rng.uniform(-1, 1, X[0].shape) * mask for _ in range(5)]
+        scale = np.abs(X[field]).max()
+        dirn = rng.uniform(-1, 1, X[0].shape) * scale
+        lamX, _ = rk.vjp(lib, X, lam)
+        adj = float(np.sum(lamX[field] * dirn))
+        # lam . F is quadratic in the momenta (a large step has no truncation
+        # error and keeps rounding small); 1/RO makes it curved in RO
+        eps = {0: 1e-4, 1: 1e-4, 2: 1e-4, 3: 1e-6, 4: 1e-5}[field]
+        Xp = [x.copy() for x in X]
+        Xm = [x.copy() for x in X]
+        Xp[field] += eps * dirn
+        Xm[field] -= eps * dirn
+        phi = lambda F: sum(float(np.sum(l * f)) for l, f in zip(lam, F))
+        fd = (phi(rk.flux(lib, Xp)) - phi(rk.flux(lib, Xm))) / (2 * eps)
+        assert abs(adj - fd) <= 1e-7 * max(abs(fd), 1e-300) + 1e-12, (r, X_NAMES[field], adj, fd)
+
+
+def test_vjp_coefficient_derivatives(lib, sim):
+    rng = np.random.default_rng(5)
+    for r in range(sim.nranks):
+        rk = Rank(sim, r)
+        X = [sim.field(r, n).copy() for n in X_NAMES]
+        mask = _interior_mask(X[0].shape)
+        lam = [rng.uniform(-1, 1, X[0].shape) * mask for _ in range(5)]
+        _, g = rk.vjp(lib, X, lam)
+        phi = lambda F: sum(float(np.sum(l * f)) for l, f in zip(lam, F))
+        ore, tk = rk.d["ore"], rk.d["ocv"] / rk.d["repr"]
+        # lam . F is linear in ore and tk: a large step is exact up to rounding
+        for idx, (vo, vt) in enumerate([((1 + 1e-3) * ore, tk), (ore, (1 + 1e-3) * tk)]):
+            lo = (2 * ore - vo, 2 * tk - vt)
+            fp = phi(rk.flux(lib, X, rk.consts(vo, vt)))
+            fm = phi(rk.flux(lib, X, rk.consts(*lo)))
+            h = (vo - ore) if idx == 0 else (vt - tk)
+            fd = (fp - fm) / (2 * h)
+            assert abs(g[idx] - fd) <= 1e-7 * abs(fd) + 1e-12, (r, idx, g[idx], fd)
