@@ -1,0 +1,50 @@
+/* r3d_adj.h - discrete adjoint of the Rast-3DMHD time step on one GPU.
+ *
+ * Extension with no counterpart in the reference.  For a run of nsteps full
+ * RK3 steps from an initial state X0 it computes the objective
+ *     J = sum over interior cells of vol * (ru^2 + rv^2 + rw^2) / (2 ro),
+ *     vol = 1 / (dxxdx[i] * dyydy[j] * dzzdz[k])   (cell volume in units of
+ *                                                   the computational cell),
+ * at the final time, and the exact gradient of that discrete J (the same
+ * floating-point program the forward path runs, differentiated by hand) with
+ * respect to every primary of X0, the Reynolds number Re and the temperature
+ * gradient theta (through the boundary heat-flux conditions).  The CFL step
+ * dt depends on the state; its sensitivity enters through the cell and limit
+ * that attain the minimum.
+ *
+ * Memory: the forward keeps a checkpoint every `checkpoint_every` steps; the
+ * backward pass recomputes the forward states of one segment at a time from
+ * its checkpoint, then the two intermediate stage states of each step.
+ *
+ * Hydrodynamic runs only (lmag = 0) and tc = 0.  Errors follow r3d_hip.h.
+ */
+#ifndef R3D_ADJ_H
+#define R3D_ADJ_H
+
+#include "r3d_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct r3d_adj_ctx r3d_adj_ctx;
+
+int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** out);
+int r3d_adj_destroy(r3d_adj_ctx* a);
+int r3d_adj_set_metrics(r3d_adj_ctx* a, const r3d_hip_metrics* m);
+/* X0: fields[R3D_RU..R3D_TT], each (npx+2)(npy+2)(npz+2) doubles, x fastest. */
+int r3d_adj_set_initial(r3d_adj_ctx* a, const double* const* fields);
+/* Run nsteps steps from X0, keeping the checkpoints; J = objective. */
+int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J);
+/* Gradient of the last forward's J: grad[R3D_RU..R3D_TT] (global arrays as
+ * X0; ghost entries zero), dJ/dRe, dJ/dtheta.  NULL outputs are skipped. */
+int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double* dJ_dtheta);
+/* Wall milliseconds of the last forward and backward calls. */
+int r3d_adj_timing(r3d_adj_ctx* a, double* forward_ms, double* backward_ms);
+/* Final state of the last forward (as r3d_hip_download_global). */
+int r3d_adj_download_final(r3d_adj_ctx* a, double* const* fields);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

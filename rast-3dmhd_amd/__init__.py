@@ -103,6 +103,16 @@ def load():
         "r3d_ens_step": [vp, ip],
         "r3d_ens_sync": [vp],
         "r3d_ens_memory": [vp, ctypes.POINTER(ctypes.c_ulonglong)],
+        # discrete adjoint (include/r3d_adj.h)
+        "r3d_adj_create": [ctypes.POINTER(Cfg), ip, ctypes.POINTER(vp)],
+        "r3d_adj_destroy": [vp],
+        "r3d_adj_set_metrics": [vp, ctypes.POINTER(Metrics)],
+        "r3d_adj_set_initial": [vp, ctypes.POINTER(dp)],
+        "r3d_adj_forward": [vp, ip, ctypes.POINTER(ctypes.c_double)],
+        "r3d_adj_backward": [vp, ctypes.POINTER(dp), ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_double)],
+        "r3d_adj_timing": [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+        "r3d_adj_download_final": [vp, ctypes.POINTER(dp)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -322,6 +332,84 @@ class Ensemble:
         b = ctypes.c_ulonglong()
         _check(load().r3d_ens_memory(self.h, ctypes.byref(b)))
         return b.value
+
+
+def cfg_with(cfg, re=None, theta=None):
+    """A copy of cfg with the Reynolds number and/or theta replaced, the
+    derived coefficients recomputed as the reference does (src/params.cpp:
+    ore = 1/Re, repr = Re*Pr)."""
+    c = Cfg()
+    ctypes.pointer(c)[0] = cfg
+    if re is not None:
+        pr = cfg.repr / cfg.re
+        c.re = float(re)
+        c.ore = 1.0 / c.re
+        c.repr = c.re * pr
+    if theta is not None:
+        c.theta = float(theta)
+    return c
+
+
+class Adjoint:
+    """Discrete adjoint of a hydrodynamic run (include/r3d_adj.h): J = final
+    kinetic energy, gradients with respect to the initial primaries, Re and
+    theta.  cfg: a Cfg (e.g. cfg_with(run.cfg(), re=...)); metrics: the run's
+    global metrics."""
+
+    def __init__(self, cfg, metrics, checkpoint_every=20):
+        L = load()
+        h = ctypes.c_void_p()
+        _check(L.r3d_adj_create(ctypes.byref(cfg), checkpoint_every, ctypes.byref(h)))
+        self.h = h
+        self.shape = (cfg.npz + 2, cfg.npy + 2, cfg.npx + 2)
+        try:
+            keep = {k: np.ascontiguousarray(metrics[k], dtype=np.float64) for k in METRIC_NAMES}
+            st = Metrics(*[_dptr(keep[k]) for k in METRIC_NAMES])
+            _check(L.r3d_adj_set_metrics(h, ctypes.byref(st)))
+        except Exception:
+            self.close()
+            raise
+
+    def close(self):
+        if self.h:
+            load().r3d_adj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_initial(self, fields):
+        arrs = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in PRIMARIES[:5]]
+        for a in arrs:
+            assert a.shape == self.shape
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 5)(*[_dptr(a) for a in arrs])
+        _check(load().r3d_adj_set_initial(self.h, ptrs))
+
+    def forward(self, nsteps):
+        J = ctypes.c_double()
+        _check(load().r3d_adj_forward(self.h, nsteps, ctypes.byref(J)))
+        return J.value
+
+    def backward(self):
+        out = {n: np.zeros(self.shape) for n in PRIMARIES[:5]}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 5)(*[_dptr(out[n]) for n in PRIMARIES[:5]])
+        dre, dth = ctypes.c_double(), ctypes.c_double()
+        _check(load().r3d_adj_backward(self.h, ptrs, ctypes.byref(dre), ctypes.byref(dth)))
+        return out, dre.value, dth.value
+
+    def timing(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        _check(load().r3d_adj_timing(self.h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
+
+    def final_state(self):
+        out = {n: np.zeros(self.shape) for n in PRIMARIES[:5]}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 5)(*[_dptr(out[n]) for n in PRIMARIES[:5]])
+        _check(load().r3d_adj_download_final(self.h, ptrs))
+        return out
 
 
 def ensemble_members(n=64, seed=3):

@@ -1,0 +1,716 @@
+// r3d_adjoint.hip - discrete adjoint of the hydrodynamic time step
+// (include/r3d_adj.h).
+//
+// Forward step n from X_n with step size dt_n = sff * min(limits(X_n)):
+//     X1 = B(X_n + g1 dt F(X_n)),   Z1 = X1 + z1 dt F(X_n)
+//     X2 = B(Z1  + g2 dt F(X1)),    Z2 = X2 + z2 dt F(X1)
+//     X_{n+1} = B(Z2 + g3 dt F(X2))
+// where B is the boundary pass (reference src/bcon.cpp) applied to the wall
+// planes.  The backward pass runs this in reverse for each step, with
+//     lamX <- B^T lamX,   lamF = g dt lamX (+ z dt lamZ),   lamX += (dF/dX)^T lamF
+// (the flux transpose is cell_flux_vjp of r3d_adj.hpp, scattered to the 19
+// stencil points), accumulates lam_dt = sum over the stages of the
+// coefficient times <F, lam>, and adds lam_dt * d(dt)/dX_n at the cell and
+// limit that attain the minimum.  Re enters through ore = 1/Re and
+// tk = ocv/(Re Pr) in the fluxes and through the thermal and viscous dt
+// limits; theta through the boundary heat-flux conditions.
+//
+// The intermediate stage states X1, X2 of a step are recomputed from X_n by
+// truncated steps (r3d_hip_step with 1 and 2 stages) of a work context; the
+// states X_n of one segment are recomputed from its checkpoint.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "r3d_adj.h"
+#include "r3d_adj.hpp"
+#include "r3d_ctx.hpp"
+
+using namespace r3d;
+
+namespace {
+
+constexpr int AT = 256;          // threads per block of the adjoint kernels
+constexpr int ADJ_BLOCKS = 2048; // grid-stride blocks of the cell kernels
+constexpr int NF = 5;            // hydrodynamic primaries
+
+__device__ __forceinline__ void cell_of(const Geo& g, long q, int& i, int& j, int& k) {
+    i = 1 + (int)(q % g.npx);
+    const long r = q / g.npx;
+    j = 1 + (int)(r % g.npy);
+    k = 1 + (int)(r / g.npy);
+}
+__device__ __forceinline__ int wrapx(const Geo& g, int i) {
+    return i == 0 ? g.npx : (i == g.npx + 1 ? 1 : i);
+}
+__device__ __forceinline__ int wrapy(const Geo& g, int j) {
+    return j == 0 ? g.npy : (j == g.npy + 1 ? 1 : j);
+}
+
+// The flux stencil of cell (i, j, k) from the global array: periodic x/y
+// neighbours are read from the cells they wrap to, as the stage kernel does.
+__device__ void gather_stencil(const Geo& g, const State& X, const Metrics& M, int i, int j, int k,
+                               Stencil& s, CellMet& m) {
+    for (int p = 0; p < NPTS; ++p) {
+        int di, dj, dk;
+        pt_offset(p, di, dj, dk);
+        const long n = g.at(wrapx(g, i + di), wrapy(g, j + dj), k + dk);
+        s.p[p] = {X.q[QRU][n], X.q[QRV][n], X.q[QRW][n], X.q[QRO][n], X.q[QTT][n]};
+    }
+    m.wall = k == 1 ? 1 : (k == g.npz ? 2 : 0);
+    s.rw2 = m.wall == 1 ? X.q[QRW][g.at(i, j, k + 2)]
+                        : (m.wall == 2 ? X.q[QRW][g.at(i, j, k - 2)] : 0.0);
+    m.dxx = M.dxx[i];
+    m.d2xx = M.d2xx[i];
+    m.dyy = M.dyy[j];
+    m.d2yy = M.d2yy[j];
+    m.dzz = M.dzz[k];
+    m.d2zz = M.d2zz[k];
+    m.rk = M.rkapa[k];
+    m.dk = M.dkapa[k];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum v over the block into out[slot] (or atomically into *acc).
+__device__ void block_sum(double v, double* red, double* acc) {
+    v = wave_sum(v);
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) red[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+        atomicAdd(acc, s);
+    }
+    __syncthreads();
+}
+
+// F of every interior cell (identical to the stage kernel's F).
+__global__ __launch_bounds__(AT) void k_flux_eval(const Geo g, const State X, const Consts C,
+                                                  const Metrics M, const State F) {
+    const long total = (long)g.npx * g.npy * g.npz;
+    for (long q = (long)blockIdx.x * AT + threadIdx.x; q < total; q += (long)gridDim.x * AT) {
+        int i, j, k;
+        cell_of(g, q, i, j, k);
+        Stencil s;
+        CellMet m;
+        gather_stencil(g, X, M, i, j, k, s, m);
+        double f[5];
+        cell_flux(s, m, C, f);
+        const long n = g.at(i, j, k);
+        for (int c = 0; c < 5; ++c) F.q[c][n] = f[c];
+    }
+}
+
+// LX += (dF/dX)^T LF; acc[0] += d(LF.F)/d(ore), acc[1] += d(LF.F)/d(tk).
+__global__ __launch_bounds__(AT) void k_flux_vjp(const Geo g, const State X, const Consts C,
+                                                 const Metrics M, const State LF, const State LX,
+                                                 double* acc) {
+    __shared__ double red[AT / 64];
+    double go = 0.0, gt = 0.0;
+    const long total = (long)g.npx * g.npy * g.npz;
+    for (long q = (long)blockIdx.x * AT + threadIdx.x; q < total; q += (long)gridDim.x * AT) {
+        int i, j, k;
+        cell_of(g, q, i, j, k);
+        const long n = g.at(i, j, k);
+        const double lam[5] = {LF.q[0][n], LF.q[1][n], LF.q[2][n], LF.q[3][n], LF.q[4][n]};
+        Stencil s, A;
+        CellMet m;
+        gather_stencil(g, X, M, i, j, k, s, m);
+        double a, b;
+        cell_flux_vjp(s, m, C, lam, A, a, b);
+        go += a;
+        gt += b;
+        for (int p = 0; p < NPTS; ++p) {
+            int di, dj, dk;
+            pt_offset(p, di, dj, dk);
+            const int kk = k + dk;
+            if (kk < 1 || kk > g.npz) continue;  // outer planes are not state
+            const long t = g.at(wrapx(g, i + di), wrapy(g, j + dj), kk);
+            atomicAdd(&LX.q[QRU][t], A.p[p].ru);
+            atomicAdd(&LX.q[QRV][t], A.p[p].rv);
+            atomicAdd(&LX.q[QRW][t], A.p[p].rw);
+            atomicAdd(&LX.q[QRO][t], A.p[p].ro);
+            atomicAdd(&LX.q[QTT][t], A.p[p].tt);
+        }
+        if (m.wall) atomicAdd(&LX.q[QRW][g.at(i, j, m.wall == 1 ? k + 2 : k - 2)], A.rw2);
+    }
+    block_sum(go, red, &acc[0]);
+    block_sum(gt, red, &acc[1]);
+}
+
+// LX <- B^T LX on the two wall planes, with X the post-boundary state;
+// acc[3] += d/d(theta).  grid = (ceil(npx*npy / AT), 2 sides).
+__global__ __launch_bounds__(AT) void k_bcon_adj(const Geo g, const State X, const Consts C,
+                                                 const Metrics M, const State LX, double* acc) {
+    __shared__ double red[AT / 64];
+    double gth = 0.0;
+    const int side = blockIdx.y;
+    const int q = blockIdx.x * AT + threadIdx.x;
+    if (q < g.npx * g.npy) {
+        const int i = 1 + q % g.npx, j = 1 + q / g.npx;
+        const int kw = side ? g.npz : 1;
+        const long n = g.at(i, j, kw);
+        const long d1 = side ? -g.pp : g.pp;
+        const long n1 = n + d1, n2 = n + 2 * d1;
+        const double ro = X.q[QRO][n], ro1 = X.q[QRO][n1], ro2 = X.q[QRO][n2];
+        for (int f = QRU; f <= QRV; ++f) {
+            // m = ro * (c43 m1 / ro1 - c13 m2 / ro2)  (src/bcon.cpp:22-37)
+            const double a = LX.q[f][n];
+            const double m1 = X.q[f][n1], m2 = X.q[f][n2];
+            LX.q[QRO][n] += a * (C.c43 * m1 / ro1 - C.c13 * m2 / ro2);
+            LX.q[f][n1] += a * ro * C.c43 / ro1;
+            LX.q[QRO][n1] -= a * ro * C.c43 * m1 / (ro1 * ro1);
+            LX.q[f][n2] -= a * ro * C.c13 / ro2;
+            LX.q[QRO][n2] += a * ro * C.c13 * m2 / (ro2 * ro2);
+            LX.q[f][n] = 0.0;
+        }
+        LX.q[QRW][n] = 0.0;
+        const double aT = LX.q[QTT][n];
+        bool fixed = false, extrap = false;
+        double dth = 0.0;
+        if (side == 0) {
+            switch (C.top_mode) {
+                case TOP_GAUSS_FLUX:
+                    extrap = true;
+                    dth = -C.c23 * M.dz_top * (1.0 - M.e1[i] / C.hh * M.e2[j] / C.hh);
+                    break;
+                case TOP_CONST_FLUX:
+                    extrap = true;
+                    dth = -C.c23 * M.dz_top;
+                    break;
+                case TOP_GAUSS_T:
+                case TOP_CONST_T:
+                    fixed = true;
+                    break;
+                default:
+                    break;
+            }
+        } else if (C.izcon == 0) {
+            fixed = true;
+        } else {
+            extrap = true;
+            dth = C.c23 * M.dz_bot / M.rkapa[kw];
+        }
+        if (extrap) {
+            LX.q[QTT][n1] += C.c43 * aT;
+            LX.q[QTT][n2] -= C.c13 * aT;
+            gth = aT * dth;
+        }
+        if (fixed || extrap) LX.q[QTT][n] = 0.0;
+    }
+    block_sum(gth, red, &acc[3]);
+}
+
+// Interior combination for one reverse stage (all five fields):
+//   s = src_z ? LZ : LX;  LF = (accumulate ? LF : 0) + a s;  LZ = LX if copy_z;
+//   acc[2] += dmul <F, s>.
+__global__ __launch_bounds__(AT) void k_lin(const Geo g, const State LX, const State LZ,
+                                            const State LF, const State F, double a, double dmul,
+                                            int src_z, int accumulate, int copy_z, double* acc) {
+    __shared__ double red[AT / 64];
+    double d = 0.0;
+    const long total = (long)g.npx * g.npy * g.npz;
+    for (long q = (long)blockIdx.x * AT + threadIdx.x; q < total; q += (long)gridDim.x * AT) {
+        int i, j, k;
+        cell_of(g, q, i, j, k);
+        const long n = g.at(i, j, k);
+        for (int f = 0; f < NF; ++f) {
+            const double s = src_z ? LZ.q[f][n] : LX.q[f][n];
+            d += F.q[f][n] * s;
+            LF.q[f][n] = (accumulate ? LF.q[f][n] : 0.0) + a * s;
+            if (copy_z) LZ.q[f][n] = LX.q[f][n];
+        }
+    }
+    block_sum(dmul * d, red, &acc[2]);
+}
+
+// Objective per block (deterministic host sum) and, if LX is given, its
+// gradient: J = sum vol (ru^2+rv^2+rw^2)/(2 ro), vol = 1/(dxx dyy dzz).
+__global__ __launch_bounds__(AT) void k_ke(const Geo g, const State X, const Metrics M,
+                                           const State LX, int want_grad, double* part) {
+    __shared__ double red[AT / 64];
+    double sum = 0.0;
+    const long total = (long)g.npx * g.npy * g.npz;
+    for (long q = (long)blockIdx.x * AT + threadIdx.x; q < total; q += (long)gridDim.x * AT) {
+        int i, j, k;
+        cell_of(g, q, i, j, k);
+        const long n = g.at(i, j, k);
+        const double vol = 1.0 / (M.dxx[i] * M.dyy[j] * M.dzz[k]);
+        const double ru = X.q[QRU][n], rv = X.q[QRV][n], rw = X.q[QRW][n], ro = X.q[QRO][n];
+        const double m2 = ru * ru + rv * rv + rw * rw;
+        sum += vol * 0.5 * m2 / ro;
+        if (want_grad) {
+            LX.q[QRU][n] = vol * ru / ro;
+            LX.q[QRV][n] = vol * rv / ro;
+            LX.q[QRW][n] = vol * rw / ro;
+            LX.q[QRO][n] = -vol * 0.5 * m2 / (ro * ro);
+            LX.q[QTT][n] = 0.0;
+        }
+    }
+    sum = wave_sum(sum);
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) red[wv] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < AT / 64; ++w) s += red[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+// Per-block minimum of the dt limits (the formulas of dt_terms) with the key
+// cell * 4 + limit, ties to the smaller key.
+__global__ __launch_bounds__(AT) void k_dt_argmin(const Geo g, const State X, const Consts C,
+                                                  const Metrics M, double* bval, long* bkey) {
+    __shared__ double rv[AT / 64];
+    __shared__ long rk[AT / 64];
+    double best = 1e300;
+    long key = LONG_MAX;
+    const long total = (long)g.npx * g.npy * g.npz;
+    for (long q = (long)blockIdx.x * AT + threadIdx.x; q < total; q += (long)gridDim.x * AT) {
+        int i, j, k;
+        cell_of(g, q, i, j, k);
+        const long n = g.at(i, j, k);
+        const double ru = X.q[QRU][n], rv_ = X.q[QRV][n], rw = X.q[QRW][n];
+        const double ro = X.q[QRO][n], tt = X.q[QTT][n];
+        const double dd = dmin(dmin(M.ddx[i], M.ddy[j]), M.ddz[k]);
+        const double uu = (ru * ru + rv_ * rv_ + rw * rw) * (1.0 / ro) * (1.0 / ro);
+        const double fast = uu + C.gamma * tt + 2.0 * sqrt(uu * C.gamma * tt);
+        const double L[3] = {dd / sqrt(fast), 0.5 * dd * dd * C.repr * C.cv * ro / M.rkapa[k],
+                             0.375 * dd * dd * C.re * ro};
+        for (int t = 0; t < 3; ++t) {
+            const long kk = q * 4 + t + 1;
+            if (L[t] < best || (L[t] == best && kk < key)) {
+                best = L[t];
+                key = kk;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const long ok = __shfl_xor(key, o, 64);
+        if (ob < best || (ob == best && ok < key)) {
+            best = ob;
+            key = ok;
+        }
+    }
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) {
+        rv[wv] = best;
+        rk[wv] = key;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < AT / 64; ++w)
+            if (rv[w] < best || (rv[w] == best && rk[w] < key)) {
+                best = rv[w];
+                key = rk[w];
+            }
+        bval[blockIdx.x] = best;
+        bkey[blockIdx.x] = key;
+    }
+}
+
+}  // namespace
+
+struct r3d_adj_ctx {
+    r3d_hip_ctx* work = nullptr;   // forward steps (plain path)
+    int K = 20;                     // checkpoint cadence
+    int nsteps = 0;
+    size_t fe = 0;                  // doubles per field array
+    std::vector<double*> ckpt;      // X at steps 0, K, 2K, ...
+    std::vector<double*> seg;       // X at the steps of one segment
+    double *x0 = nullptr, *xfinal = nullptr, *x1b = nullptr, *x2b = nullptr;
+    double *f0 = nullptr, *f1 = nullptr, *f2 = nullptr;
+    double *lx = nullptr, *lz = nullptr, *lf = nullptr;
+    double* acc = nullptr;          // g_ore, g_tk, lam_dt, g_theta
+    double* part = nullptr;         // k_ke block sums
+    double* bval = nullptr;
+    long* bkey = nullptr;
+    double J = 0.0;
+    double fwd_ms = 0.0, bwd_ms = 0.0;
+    unsigned long long bytes = 0;
+};
+
+namespace {
+
+State state_of(const r3d_adj_ctx* a, double* base) {
+    State s{};
+    for (int f = 0; f < NF; ++f) s.q[f] = base + f * a->fe;
+    return s;
+}
+
+int alloc_state(r3d_adj_ctx* a, double** p) {
+    if (hipMalloc((void**)p, a->fe * NF * sizeof(double)) != hipSuccess) {
+        *p = nullptr;
+        return set_err(R3D_ENOMEM, "adjoint: state buffer of %zu bytes",
+                       a->fe * NF * sizeof(double));
+    }
+    a->bytes += a->fe * NF * sizeof(double);
+    return R3D_OK;
+}
+
+// work context's current X <-> a state buffer
+int put_state(r3d_adj_ctx* a, const double* src) {
+    r3d_hip_ctx* w = a->work;
+    for (int f = 0; f < NF; ++f)
+        HIPCHK(hipMemcpyAsync(w->xs[w->cur][f], src + f * a->fe, a->fe * sizeof(double),
+                              hipMemcpyDeviceToDevice, w->stream));
+    w->next_valid = false;
+    return R3D_OK;
+}
+int get_state(r3d_adj_ctx* a, double* dst) {
+    r3d_hip_ctx* w = a->work;
+    for (int f = 0; f < NF; ++f)
+        HIPCHK(hipMemcpyAsync(dst + f * a->fe, w->xs[w->cur][f], a->fe * sizeof(double),
+                              hipMemcpyDeviceToDevice, w->stream));
+    return R3D_OK;
+}
+
+int blocks_for(const Geo& g) {
+    const long total = (long)g.npx * g.npy * g.npz;
+    return (int)std::min<long>(ADJ_BLOCKS, (total + AT - 1) / AT);
+}
+
+int objective(r3d_adj_ctx* a, const double* x, bool grad, double* J) {
+    r3d_hip_ctx* w = a->work;
+    const int nb = blocks_for(w->g);
+    hipLaunchKernelGGL(k_ke, dim3(nb), dim3(AT), 0, w->stream, w->g, state_of(a, (double*)x),
+                       w->M, state_of(a, a->lx), grad ? 1 : 0, a->part);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(nb);
+    HIPCHK(hipMemcpyAsync(h.data(), a->part, nb * sizeof(double), hipMemcpyDeviceToHost,
+                          w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    double s = 0.0;
+    for (double v : h) s += v;
+    *J = s;
+    return R3D_OK;
+}
+
+// One truncated forward step from x (1 or 2 stages) into out; returns dt.
+int stage_state(r3d_adj_ctx* a, const double* x, int nstages, double* out, double* dt) {
+    int rc;
+    if ((rc = put_state(a, x))) return rc;
+    if ((rc = r3d_hip_step(a->work, 1, nstages))) return rc;
+    if ((rc = get_state(a, out))) return rc;
+    r3d_hip_scalars s{};
+    if ((rc = r3d_hip_get_scalars(a->work, &s))) return rc;
+    *dt = s.dt;
+    return R3D_OK;
+}
+
+int flux(r3d_adj_ctx* a, const double* x, double* f) {
+    r3d_hip_ctx* w = a->work;
+    hipLaunchKernelGGL(k_flux_eval, dim3(blocks_for(w->g)), dim3(AT), 0, w->stream, w->g,
+                       state_of(a, (double*)x), w->C, w->M, state_of(a, f));
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
+}
+
+int vjp(r3d_adj_ctx* a, const double* x) {
+    r3d_hip_ctx* w = a->work;
+    hipLaunchKernelGGL(k_flux_vjp, dim3(blocks_for(w->g)), dim3(AT), 0, w->stream, w->g,
+                       state_of(a, (double*)x), w->C, w->M, state_of(a, a->lf),
+                       state_of(a, a->lx), a->acc);
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
+}
+
+int bcon_adj(r3d_adj_ctx* a, const double* x) {
+    r3d_hip_ctx* w = a->work;
+    const int nb = (w->g.npx * w->g.npy + AT - 1) / AT;
+    hipLaunchKernelGGL(k_bcon_adj, dim3(nb, 2), dim3(AT), 0, w->stream, w->g,
+                       state_of(a, (double*)x), w->C, w->M, state_of(a, a->lx), a->acc);
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
+}
+
+int lin(r3d_adj_ctx* a, const double* F, double coef, double dmul, int src_z, int accumulate,
+        int copy_z) {
+    r3d_hip_ctx* w = a->work;
+    hipLaunchKernelGGL(k_lin, dim3(blocks_for(w->g)), dim3(AT), 0, w->stream, w->g,
+                       state_of(a, a->lx), state_of(a, a->lz), state_of(a, a->lf),
+                       state_of(a, (double*)F), coef, dmul, src_z, accumulate, copy_z, a->acc);
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
+}
+
+// lam_dt * d(dt)/dX at the cell and limit attaining the minimum of the dt
+// limits of x; returns d(dt)/dRe * lam_dt in *gre.
+int dt_adjoint(r3d_adj_ctx* a, const double* x, double lam_dt, double* gre) {
+    r3d_hip_ctx* w = a->work;
+    const Geo& g = w->g;
+    const int nb = blocks_for(g);
+    hipLaunchKernelGGL(k_dt_argmin, dim3(nb), dim3(AT), 0, w->stream, g, state_of(a, (double*)x),
+                       w->C, w->M, a->bval, a->bkey);
+    HIPCHK(hipGetLastError());
+    std::vector<double> bv(nb);
+    std::vector<long> bk(nb);
+    HIPCHK(hipMemcpyAsync(bv.data(), a->bval, nb * sizeof(double), hipMemcpyDeviceToHost,
+                          w->stream));
+    HIPCHK(hipMemcpyAsync(bk.data(), a->bkey, nb * sizeof(long), hipMemcpyDeviceToHost,
+                          w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    double best = 1e300;
+    long key = LONG_MAX;
+    for (int b = 0; b < nb; ++b)
+        if (bv[b] < best || (bv[b] == best && bk[b] < key)) {
+            best = bv[b];
+            key = bk[b];
+        }
+    const long q = key / 4;
+    const int t = (int)(key % 4);
+    const int i = 1 + (int)(q % g.npx), j = 1 + (int)((q / g.npx) % g.npy);
+    const int k = 1 + (int)(q / g.npx / g.npy);
+    const long n = g.at(i, j, k);
+    double v[NF], dv[NF] = {0, 0, 0, 0, 0};
+    for (int f = 0; f < NF; ++f)
+        HIPCHK(hipMemcpy(&v[f], x + f * a->fe + n, sizeof(double), hipMemcpyDeviceToHost));
+    const double ru = v[QRU], rv = v[QRV], rw = v[QRW], ro = v[QRO], tt = v[QTT];
+    const Consts& C = w->C;
+    const double L = best, s = C.sff * lam_dt;
+    double dre = 0.0;
+    if (t == 1) {
+        // L = dd / sqrt(fast), fast = uu + gT + 2 sqrt(uu gT), uu = |m|^2 / ro^2
+        const double m2 = ru * ru + rv * rv + rw * rw;
+        const double uu = m2 / (ro * ro), gT = C.gamma * tt;
+        const double fast = uu + gT + 2.0 * sqrt(uu * gT);
+        const double dLdf = -0.5 * L / fast;
+        const double sq = sqrt(uu * gT);
+        const double dfdu = uu > 0.0 ? 1.0 + gT / sq : 0.0;  // u = 0: velocity derivative 0
+        const double dfdT = C.gamma * (1.0 + (uu > 0.0 ? uu / sq : 0.0));
+        dv[QRU] = dLdf * dfdu * 2.0 * ru / (ro * ro);
+        dv[QRV] = dLdf * dfdu * 2.0 * rv / (ro * ro);
+        dv[QRW] = dLdf * dfdu * 2.0 * rw / (ro * ro);
+        dv[QRO] = dLdf * dfdu * (-2.0 * uu / ro);
+        dv[QTT] = dLdf * dfdT;
+    } else {
+        // thermal 0.5 dd^2 Re Pr cv ro / rk, viscous 0.375 dd^2 Re ro: both L ~ Re ro
+        dv[QRO] = L / ro;
+        dre = L / C.re;
+    }
+    for (int f = 0; f < NF; ++f) {
+        double cur = 0.0;
+        double* p = a->lx + f * a->fe + n;
+        HIPCHK(hipMemcpy(&cur, p, sizeof(double), hipMemcpyDeviceToHost));
+        cur += s * dv[f];
+        HIPCHK(hipMemcpy(p, &cur, sizeof(double), hipMemcpyHostToDevice));
+    }
+    *gre = s * dre;
+    return R3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** out) {
+    if (!cfg || !out || checkpoint_every < 1) return set_err(R3D_EINVAL, "bad arguments");
+    *out = nullptr;
+    if (cfg->lmag) return set_err(R3D_EUNSUP, "adjoint: LMAG runs are not supported");
+    if (cfg->tc != 0.0) return set_err(R3D_EUNSUP, "adjoint: tc != 0 is not supported");
+    auto* a = new r3d_adj_ctx();
+    a->K = checkpoint_every;
+    int rc = r3d_hip_create(cfg, &a->work);
+    if (rc) {
+        delete a;
+        return rc;
+    }
+    std::fill(a->work->uploaded.begin(), a->work->uploaded.end(), 1);
+    a->fe = field_elems(a->work->g);
+    auto fail = [&](int code) {
+        r3d_adj_destroy(a);
+        return code;
+    };
+    double** bufs[] = {&a->x0, &a->xfinal, &a->x1b, &a->x2b, &a->f0, &a->f1, &a->f2,
+                       &a->lx, &a->lz, &a->lf};
+    for (double** b : bufs)
+        if ((rc = alloc_state(a, b))) return fail(rc);
+    if (hipMalloc((void**)&a->acc, 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&a->part, ADJ_BLOCKS * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&a->bval, ADJ_BLOCKS * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&a->bkey, ADJ_BLOCKS * sizeof(long)) != hipSuccess)
+        return fail(set_err(R3D_ENOMEM, "adjoint: small buffers"));
+    a->seg.assign(a->K, nullptr);
+    for (auto& s : a->seg)
+        if ((rc = alloc_state(a, &s))) return fail(rc);
+    // zero everything once: ghost entries of the adjoint fields stay zero
+    double* all[] = {a->x0, a->xfinal, a->x1b, a->x2b, a->f0, a->f1, a->f2, a->lx, a->lz, a->lf};
+    for (double* p : all)
+        if (hipMemset(p, 0, a->fe * NF * sizeof(double)) != hipSuccess)
+            return fail(set_err(R3D_EHIP, "adjoint: memset"));
+    *out = a;
+    return R3D_OK;
+}
+
+int r3d_adj_destroy(r3d_adj_ctx* a) {
+    if (!a) return R3D_OK;
+    if (a->work) (void)hipStreamSynchronize(a->work->stream);
+    double* all[] = {a->x0, a->xfinal, a->x1b, a->x2b, a->f0, a->f1, a->f2, a->lx, a->lz, a->lf,
+                     a->acc, a->part, a->bval};
+    for (double* p : all)
+        if (p) (void)hipFree(p);
+    if (a->bkey) (void)hipFree(a->bkey);
+    for (double* p : a->ckpt)
+        if (p) (void)hipFree(p);
+    for (double* p : a->seg)
+        if (p) (void)hipFree(p);
+    if (a->work) r3d_hip_destroy(a->work);
+    delete a;
+    return R3D_OK;
+}
+
+int r3d_adj_set_metrics(r3d_adj_ctx* a, const r3d_hip_metrics* m) {
+    if (!a) return set_err(R3D_EINVAL, "null adjoint context");
+    return r3d_hip_set_metrics(a->work, m);
+}
+
+int r3d_adj_set_initial(r3d_adj_ctx* a, const double* const* fields) {
+    if (!a || !fields) return set_err(R3D_EINVAL, "null argument");
+    r3d_hip_ctx* w = a->work;
+    const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
+    for (int f = 0; f < NF; ++f) {
+        if (!fields[f]) return set_err(R3D_EINVAL, "field %d missing", f);
+        HIPCHK(hipMemcpy2DAsync(a->x0 + f * a->fe + XOFF, w->g.px * sizeof(double), fields[f],
+                                W * sizeof(double), W * sizeof(double), rows,
+                                hipMemcpyHostToDevice, w->stream));
+    }
+    HIPCHK(hipStreamSynchronize(w->stream));
+    a->nsteps = 0;
+    return R3D_OK;
+}
+
+int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J) {
+    if (!a || nsteps < 1) return set_err(R3D_EINVAL, "bad arguments");
+    r3d_hip_ctx* w = a->work;
+    int rc;
+    if (!w->metrics_set) return set_err(R3D_ESTATE, "metrics not uploaded");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int nck = (nsteps + a->K - 1) / a->K;
+    while ((int)a->ckpt.size() < nck) {
+        double* p = nullptr;
+        if ((rc = alloc_state(a, &p))) return rc;
+        a->ckpt.push_back(p);
+    }
+    w->cur = 0;
+    if ((rc = put_state(a, a->x0))) return rc;
+    r3d_hip_scalars s0{};
+    if ((rc = r3d_hip_set_scalars(w, &s0))) return rc;
+    for (int n = 0; n < nsteps; ++n) {
+        if (n % a->K == 0 && (rc = get_state(a, a->ckpt[n / a->K]))) return rc;
+        if ((rc = r3d_hip_step(w, 1, 3))) return rc;
+    }
+    if ((rc = get_state(a, a->xfinal))) return rc;
+    if ((rc = objective(a, a->xfinal, false, &a->J))) return rc;
+    a->nsteps = nsteps;
+    if (J) *J = a->J;
+    a->fwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+    return R3D_OK;
+}
+
+int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double* dJ_dth) {
+    if (!a) return set_err(R3D_EINVAL, "null adjoint context");
+    if (a->nsteps < 1) return set_err(R3D_ESTATE, "no forward run to differentiate");
+    r3d_hip_ctx* w = a->work;
+    const Consts& C = w->C;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    double Jdummy;
+    if ((rc = objective(a, a->xfinal, true, &Jdummy))) return rc;  // LX = dJ/dX_N
+    double g_re = 0.0, g_th = 0.0;
+    const int N = a->nsteps, nseg = (N + a->K - 1) / a->K;
+    for (int sgm = nseg - 1; sgm >= 0; --sgm) {
+        const int n0 = sgm * a->K, n1 = std::min(N, n0 + a->K);
+        // states X_n0 .. X_{n1-1} of this segment
+        HIPCHK(hipMemcpyAsync(a->seg[0], a->ckpt[sgm], a->fe * NF * sizeof(double),
+                              hipMemcpyDeviceToDevice, w->stream));
+        if ((rc = put_state(a, a->ckpt[sgm]))) return rc;
+        for (int n = n0 + 1; n < n1; ++n) {
+            if ((rc = r3d_hip_step(w, 1, 3))) return rc;
+            if ((rc = get_state(a, a->seg[n - n0]))) return rc;
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            const double* xn = a->seg[n - n0];
+            const double* xn1 =
+                n + 1 < n1 ? a->seg[n + 1 - n0] : (n1 == N ? a->xfinal : a->ckpt[sgm + 1]);
+            double dt = 0.0, dt2 = 0.0;
+            if ((rc = stage_state(a, xn, 1, a->x1b, &dt))) return rc;
+            if ((rc = stage_state(a, xn, 2, a->x2b, &dt2))) return rc;
+            if ((rc = flux(a, xn, a->f0)) || (rc = flux(a, a->x1b, a->f1)) ||
+                (rc = flux(a, a->x2b, a->f2)))
+                return rc;
+            HIPCHK(hipMemsetAsync(a->acc, 0, 4 * sizeof(double), w->stream));
+            const double g1 = C.gam[0], g2 = C.gam[1], g3 = C.gam[2];
+            const double z1 = C.zeta[0], z2 = C.zeta[1];
+            // stage 3
+            if ((rc = bcon_adj(a, xn1))) return rc;
+            if ((rc = lin(a, a->f2, g3 * dt, g3, 0, 0, 1))) return rc;  // LF = g3 dt LX, LZ = LX
+            if ((rc = vjp(a, a->x2b))) return rc;                       // LX = LZ + J^T LF
+            if ((rc = lin(a, a->f1, z2 * dt, z2, 1, 0, 0))) return rc;  // LF = z2 dt LZ
+            // stage 2
+            if ((rc = bcon_adj(a, a->x2b))) return rc;
+            if ((rc = lin(a, a->f1, g2 * dt, g2, 0, 1, 1))) return rc;  // LF += g2 dt LX, LZ = LX
+            if ((rc = vjp(a, a->x1b))) return rc;
+            if ((rc = lin(a, a->f0, z1 * dt, z1, 1, 0, 0))) return rc;  // LF = z1 dt LZ
+            // stage 1
+            if ((rc = bcon_adj(a, a->x1b))) return rc;
+            if ((rc = lin(a, a->f0, g1 * dt, g1, 0, 1, 0))) return rc;  // LF += g1 dt LX
+            if ((rc = vjp(a, xn))) return rc;
+            double acc[4];
+            HIPCHK(hipMemcpyAsync(acc, a->acc, sizeof acc, hipMemcpyDeviceToHost, w->stream));
+            HIPCHK(hipStreamSynchronize(w->stream));
+            // ore = 1/Re, tk = ocv/(Re Pr)
+            g_re += acc[0] * (-C.ore * C.ore) + acc[1] * (-C.tk / C.re);
+            g_th += acc[3];
+            double gdt = 0.0;
+            if ((rc = dt_adjoint(a, xn, acc[2], &gdt))) return rc;
+            g_re += gdt;
+            (void)dt2;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(w->stream));
+    if (grad) {
+        const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
+        for (int f = 0; f < NF; ++f)
+            if (grad[f])
+                HIPCHK(hipMemcpy2D(grad[f], W * sizeof(double), a->lx + f * a->fe + XOFF,
+                                   w->g.px * sizeof(double), W * sizeof(double), rows,
+                                   hipMemcpyDeviceToHost));
+    }
+    if (dJ_dre) *dJ_dre = g_re;
+    if (dJ_dth) *dJ_dth = g_th;
+    a->bwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+    return R3D_OK;
+}
+
+int r3d_adj_timing(r3d_adj_ctx* a, double* fwd, double* bwd) {
+    if (!a) return set_err(R3D_EINVAL, "null adjoint context");
+    if (fwd) *fwd = a->fwd_ms;
+    if (bwd) *bwd = a->bwd_ms;
+    return R3D_OK;
+}
+
+int r3d_adj_download_final(r3d_adj_ctx* a, double* const* fields) {
+    if (!a || !fields) return set_err(R3D_EINVAL, "null argument");
+    r3d_hip_ctx* w = a->work;
+    const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
+    for (int f = 0; f < NF; ++f)
+        if (fields[f])
+            HIPCHK(hipMemcpy2D(fields[f], W * sizeof(double), a->xfinal + f * a->fe + XOFF,
+                               w->g.px * sizeof(double), W * sizeof(double), rows,
+                               hipMemcpyDeviceToHost));
+    return R3D_OK;
+}
+
+}  // extern "C"

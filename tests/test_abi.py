@@ -19,6 +19,7 @@ HEADERS = {
     "r3d_hip.h": os.path.join(ROOT, "rast-3dmhd_amd", "lib", "librast3dmhd_hip.so"),
     "r3d_host.h": os.path.join(ROOT, "rast-3dmhd_amd", "lib", "librast3dmhd_host.so"),
     "r3d_ens.h": os.path.join(ROOT, "rast-3dmhd_amd", "lib", "librast3dmhd_hip.so"),
+    "r3d_adj.h": os.path.join(ROOT, "rast-3dmhd_amd", "lib", "librast3dmhd_hip.so"),
 }
 
 
