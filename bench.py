@@ -95,6 +95,36 @@ def load_package():
     return mod
 
 
+def run_adjoint(a):
+    """Adjoint:forward time ratio of the hand-written discrete adjoint."""
+    pkg = load_package()
+    args = ["--npx", str(a.npx), "--npy", str(a.npy), "--npz", str(a.npz),
+            "--npey", str(a.npey), "--npez", str(a.npez)]
+    run = pkg.Run(args)
+    cells = a.npx * a.npy * a.npz
+    adj = pkg.Adjoint(run.cfg(), run.metrics(), 20)
+    adj.set_initial(run.state())
+    log(f"[bench] adjoint: {a.steps} steps of {args}")
+    adj.forward(min(a.steps, 2))  # warm-up (kernel loading)
+    J = adj.forward(a.steps)
+    adj.backward()
+    fwd_ms, bwd_ms = adj.timing()
+    log(f"[bench] J = {J:.9e}; forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms")
+    out = {
+        "metric": METRIC, "value": bwd_ms / fwd_ms, "unit": "adjoint:forward time ratio",
+        "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": bwd_ms / a.steps,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: the reference's static polytrope with its seeded ran2 perturbation",
+        "config": {"workload": f"hydro adjoint {a.npx}x{a.npy}x{a.npz}, checkpoints every 20 "
+                               "steps, J = final kinetic energy",
+                   "global_cells": cells, "forward_ms": fwd_ms, "backward_ms": bwd_ms,
+                   "forward_cell_updates_per_s": cells * 3 * a.steps / (fwd_ms / 1e3)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_ensemble(a):
     """Aggregate cell-updates/s of the batched ensemble (autotuned tiles, graph
     replay) next to the same members stepped one after another."""
@@ -165,6 +195,12 @@ def main():
                     help="batched ensemble of M seeded hydro members (sizes 48..96 x 96..192, "
                          "Re in [1,4], theta in [0.1,0.3]) against the same members run one "
                          "after another through the plain path")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the discrete adjoint (J = final kinetic energy, checkpoints every "
+                         "20 steps) against its forward run; use with --npx 128 --npy 128 "
+                         "--npz 256 --npey 2 --npez 4 --steps 200")
+    ap.add_argument("--variant", type=int, default=0, choices=(0, 1),
+                    help="0: bit-exact stage kernel (default); 1: FMA-contracted stage kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="before the run, time one device-to-device copy of this many GiB "
@@ -177,6 +213,8 @@ def main():
 
     if a.ensemble:
         return run_ensemble(a)
+    if a.adjoint:
+        return run_adjoint(a)
     extra = MHD_FLAGS if a.mhd else []
     bpc = BYTES_PER_CELL_STAGE_MHD if a.mhd else BYTES_PER_CELL_STAGE
     cpu = None if (a.no_cpu_baseline or a.mhd) else cpu_baseline(a.npx, a.npy, a.npey, a.npez,
@@ -200,7 +238,7 @@ def main():
     t0 = time.perf_counter()
     run = pkg.Run(args)
     log(f"[bench] host setup done in {time.perf_counter() - t0:.1f}s; uploading")
-    sim = run.make_sim()
+    sim = run.make_sim(a.variant)
     run.close()
     mem = sim.memory_bytes()
     log(f"[bench] device state {mem / 2**30:.1f} GiB; warm-up {a.warmup} steps")
@@ -252,7 +290,8 @@ def main():
                 "temperature perturbation (no dataset involved)",
         "config": {"workload": f"{'rotating MHD' if a.mhd else 'hydro'} "
                                f"{a.npx}x{a.npy}x{a.npz}, virtual "
-                               f"{a.npey}x{a.npez} subdomains on one GPU, bit-exact variant",
+                               f"{a.npey}x{a.npez} subdomains on one GPU, "
+                               f"{'FMA-contracted' if a.variant else 'bit-exact'} variant",
                    "global_cells": cells, "stages_per_step": 3,
                    "device_bytes": mem, "hbm_roofline_frac_of_value":
                        value * bpc / (HBM_PEAK_GBS * 1e9)},

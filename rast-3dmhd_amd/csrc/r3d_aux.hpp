@@ -17,7 +17,7 @@
 
 #include "r3d_stage.hpp"
 
-namespace r3d {
+namespace R3D_KNS {
 
 constexpr int BOUND_THREADS = 256;
 constexpr int DT_THREADS = 256;
@@ -210,4 +210,4 @@ __device__ __forceinline__ void finish_block(DevScal* sc, const double* partial,
 }
 
 
-}  // namespace r3d
+}  // namespace R3D_KNS

@@ -72,5 +72,8 @@ int dev_alloc(r3d_hip_ctx* c, double** p, size_t n);
 int fill_ghosts(r3d_hip_ctx* c);
 // All virtual ranks and the metrics uploaded.
 int check_ready(r3d_hip_ctx* c);
+// The FMA-contracted stage kernel (r3d_fma.hip); ty must be the run's tile rows.
+hipError_t launch_stage_fma(const StageArgs& a, const Consts& C, const Metrics& M, int ty,
+                            bool mag, unsigned grid, hipStream_t stream);
 
 }  // namespace r3d

@@ -221,7 +221,9 @@ int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     else if (stage == 1 && zmode == 0) R3D_STAGE(TY, 1, 0, MG); \
     else if (stage == 1) R3D_STAGE(TY, 1, 2, MG);               \
     else R3D_STAGE(TY, 2, 2, MG);
-        if (c->cfg.lmag) {
+        if (c->cfg.variant == 1) {
+            (void)launch_stage_fma(a, c->C, c->M, c->ty, c->cfg.lmag != 0, c->grid, c->stream);
+        } else if (c->cfg.lmag) {
             R3D_STAGES(TILE_Y_MAG, true)
         } else {
             R3D_STAGES(TILE_Y, false)
@@ -290,7 +292,8 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     if (f.izcon != 0 && f.izcon != 1) return set_err(R3D_EINVAL, "BCON: invalid IZCON %d", f.izcon);
     if (f.itcon == 2 || f.itcon == 3)
         return set_err(R3D_EUNSUP, "FLUXES: unsupported feature enabled: ITCON embedded heat/thermal");
-    if (f.variant != 0) return set_err(R3D_EUNSUP, "only the bit-exact variant is built");
+    if (f.variant != 0 && f.variant != 1)
+        return set_err(R3D_EUNSUP, "variant %d: 0 (bit-exact) and 1 (FMA) are built", f.variant);
     HIPCHK(hipSetDevice(f.device));
 
     auto* c = new r3d_hip_ctx();

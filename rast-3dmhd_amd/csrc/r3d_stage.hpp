@@ -38,7 +38,15 @@
 
 #include "r3d_dev.hpp"
 
-namespace r3d {
+// The kernels live in namespace R3D_KNS: r3d for the bit-exact build; the
+// FMA-contracted variant compiles this same code a second time, in its own
+// namespace and translation unit, with -ffp-contract=fast (r3d_fma.hip).
+#ifndef R3D_KNS
+#define R3D_KNS r3d
+#endif
+
+namespace R3D_KNS {
+using namespace r3d;
 
 // Quantities a column publishes to its in-plane neighbours.
 enum { SH_RU = 0, SH_RV, SH_RW, SH_TT, SH_P, SH_U, SH_V, SH_W, SH_WZ, SH_UZ, SH_VZ, NSH,
@@ -615,4 +623,4 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, cons
     stage_tile<TY, STAGE, ZMODE, MAG>(a, C, M, t);
 }
 
-}  // namespace r3d
+}  // namespace R3D_KNS
