@@ -9,15 +9,16 @@
 // planes.  The backward pass runs this in reverse for each step, with
 //     lamX <- B^T lamX,   lamF = g dt lamX (+ z dt lamZ),   lamX += (dF/dX)^T lamF
 // (the flux transpose is cell_flux_vjp of r3d_adj.hpp, scattered to the 19
-// stencil points), accumulates lam_dt = sum over the stages of the
-// coefficient times <F, lam>, and adds lam_dt * d(dt)/dX_n at the cell and
-// limit that attain the minimum.  Re enters through ore = 1/Re and
-// tk = ocv/(Re Pr) in the fluxes and through the thermal and viscous dt
-// limits; theta through the boundary heat-flux conditions.
+// stencil points), accumulates lam_dt = sum over the stages of <F, lamF>/dt,
+// and adds lam_dt * d(dt)/dX_n at the cell and limit that attain the
+// minimum.  Re enters through ore = 1/Re and tk = ocv/(Re Pr) in the fluxes
+// and through the thermal and viscous dt limits; theta through the boundary
+// heat-flux conditions.
 //
-// The intermediate stage states X1, X2 of a step are recomputed from X_n by
-// truncated steps (r3d_hip_step with 1 and 2 stages) of a work context; the
-// states X_n of one segment are recomputed from its checkpoint.
+// The states X_n, X1, X2 of every step of one segment are recomputed from
+// the segment's checkpoint by one forward pass that keeps the stage states.
+// Three adjoint buffers rotate through the stages, so that every flux
+// transpose reads its inputs from buffers it does not write.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -92,58 +93,58 @@ __device__ void block_sum(double v, double* red, double* acc) {
     __syncthreads();
 }
 
-// F of every interior cell (identical to the stage kernel's F).
-__global__ __launch_bounds__(AT) void k_flux_eval(const Geo g, const State X, const Consts C,
-                                                  const Metrics M, const State F) {
-    const long total = (long)g.npx * g.npy * g.npz;
-    for (long q = (long)blockIdx.x * AT + threadIdx.x; q < total; q += (long)gridDim.x * AT) {
-        int i, j, k;
-        cell_of(g, q, i, j, k);
-        Stencil s;
-        CellMet m;
-        gather_stencil(g, X, M, i, j, k, s, m);
-        double f[5];
-        cell_flux(s, m, C, f);
-        const long n = g.at(i, j, k);
-        for (int c = 0; c < 5; ++c) F.q[c][n] = f[c];
-    }
-}
-
-// LX += (dF/dX)^T LF; acc[0] += d(LF.F)/d(ore), acc[1] += d(LF.F)/d(tk).
+// One reverse stage: lamF = ca * LA + cb * LB at every interior cell, then
+// LO += LB (the stage input's own adjoint, passed through) + (dF/dX)^T lamF.
+// acc[0] += d(lamF.F)/d(ore), acc[1] += d(lamF.F)/d(tk), acc[2] += <F, lamF>.
+// LO must not alias LA or LB.
 __global__ __launch_bounds__(AT) void k_flux_vjp(const Geo g, const State X, const Consts C,
-                                                 const Metrics M, const State LF, const State LX,
+                                                 const Metrics M, const State LA, double ca,
+                                                 const State LB, double cb, const State LO,
                                                  double* acc) {
     __shared__ double red[AT / 64];
-    double go = 0.0, gt = 0.0;
+    double go = 0.0, gt = 0.0, dot = 0.0;
     const long total = (long)g.npx * g.npy * g.npz;
     for (long q = (long)blockIdx.x * AT + threadIdx.x; q < total; q += (long)gridDim.x * AT) {
         int i, j, k;
         cell_of(g, q, i, j, k);
         const long n = g.at(i, j, k);
-        const double lam[5] = {LF.q[0][n], LF.q[1][n], LF.q[2][n], LF.q[3][n], LF.q[4][n]};
+        double lam[5], own[5];
+        for (int f = 0; f < 5; ++f) {
+            own[f] = LB.q[f][n];
+            lam[f] = cb * own[f] + (ca != 0.0 ? ca * LA.q[f][n] : 0.0);
+        }
         Stencil s, A;
         CellMet m;
         gather_stencil(g, X, M, i, j, k, s, m);
+        double F[5];
+        cell_flux(s, m, C, F);
+        for (int f = 0; f < 5; ++f) dot += F[f] * lam[f];
         double a, b;
         cell_flux_vjp(s, m, C, lam, A, a, b);
         go += a;
         gt += b;
+        A.p[P_C].ru += own[QRU];
+        A.p[P_C].rv += own[QRV];
+        A.p[P_C].rw += own[QRW];
+        A.p[P_C].ro += own[QRO];
+        A.p[P_C].tt += own[QTT];
         for (int p = 0; p < NPTS; ++p) {
             int di, dj, dk;
             pt_offset(p, di, dj, dk);
             const int kk = k + dk;
             if (kk < 1 || kk > g.npz) continue;  // outer planes are not state
             const long t = g.at(wrapx(g, i + di), wrapy(g, j + dj), kk);
-            atomicAdd(&LX.q[QRU][t], A.p[p].ru);
-            atomicAdd(&LX.q[QRV][t], A.p[p].rv);
-            atomicAdd(&LX.q[QRW][t], A.p[p].rw);
-            atomicAdd(&LX.q[QRO][t], A.p[p].ro);
-            atomicAdd(&LX.q[QTT][t], A.p[p].tt);
+            atomicAdd(&LO.q[QRU][t], A.p[p].ru);
+            atomicAdd(&LO.q[QRV][t], A.p[p].rv);
+            atomicAdd(&LO.q[QRW][t], A.p[p].rw);
+            atomicAdd(&LO.q[QRO][t], A.p[p].ro);
+            atomicAdd(&LO.q[QTT][t], A.p[p].tt);
         }
-        if (m.wall) atomicAdd(&LX.q[QRW][g.at(i, j, m.wall == 1 ? k + 2 : k - 2)], A.rw2);
+        if (m.wall) atomicAdd(&LO.q[QRW][g.at(i, j, m.wall == 1 ? k + 2 : k - 2)], A.rw2);
     }
     block_sum(go, red, &acc[0]);
     block_sum(gt, red, &acc[1]);
+    block_sum(dot, red, &acc[2]);
 }
 
 // LX <- B^T LX on the two wall planes, with X the post-boundary state;
@@ -207,29 +208,6 @@ __global__ __launch_bounds__(AT) void k_bcon_adj(const Geo g, const State X, con
         if (fixed || extrap) LX.q[QTT][n] = 0.0;
     }
     block_sum(gth, red, &acc[3]);
-}
-
-// Interior combination for one reverse stage (all five fields):
-//   s = src_z ? LZ : LX;  LF = (accumulate ? LF : 0) + a s;  LZ = LX if copy_z;
-//   acc[2] += dmul <F, s>.
-__global__ __launch_bounds__(AT) void k_lin(const Geo g, const State LX, const State LZ,
-                                            const State LF, const State F, double a, double dmul,
-                                            int src_z, int accumulate, int copy_z, double* acc) {
-    __shared__ double red[AT / 64];
-    double d = 0.0;
-    const long total = (long)g.npx * g.npy * g.npz;
-    for (long q = (long)blockIdx.x * AT + threadIdx.x; q < total; q += (long)gridDim.x * AT) {
-        int i, j, k;
-        cell_of(g, q, i, j, k);
-        const long n = g.at(i, j, k);
-        for (int f = 0; f < NF; ++f) {
-            const double s = src_z ? LZ.q[f][n] : LX.q[f][n];
-            d += F.q[f][n] * s;
-            LF.q[f][n] = (accumulate ? LF.q[f][n] : 0.0) + a * s;
-            if (copy_z) LZ.q[f][n] = LX.q[f][n];
-        }
-    }
-    block_sum(dmul * d, red, &acc[2]);
 }
 
 // Objective per block (deterministic host sum) and, if LX is given, its
@@ -319,6 +297,61 @@ __global__ __launch_bounds__(AT) void k_dt_argmin(const Geo g, const State X, co
     }
 }
 
+// Single block: the cell and limit attaining the minimum over the block
+// results, then LX[cell] += sff * lam_dt * dL/dX there, lam_dt = acc[2] / dt,
+// and acc[4] += sff * lam_dt * dL/dRe.  At a cell at rest the advective
+// limit's velocity derivative is taken as 0.
+__global__ __launch_bounds__(64) void k_dt_apply(const Geo g, const State X, const Consts C,
+                                                 const double* bval, const long* bkey, int nb,
+                                                 double dt, const State LX, double* acc) {
+    double best = 1e300;
+    long key = LONG_MAX;
+    for (int b = threadIdx.x; b < nb; b += 64)
+        if (bval[b] < best || (bval[b] == best && bkey[b] < key)) {
+            best = bval[b];
+            key = bkey[b];
+        }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const long ok = __shfl_xor(key, o, 64);
+        if (ob < best || (ob == best && ok < key)) {
+            best = ob;
+            key = ok;
+        }
+    }
+    if (threadIdx.x != 0) return;
+    const long q = (key - 1) / 4;
+    const int t = (int)((key - 1) % 4);  // 0 advective, 1 thermal, 2 viscous
+    int i, j, k;
+    cell_of(g, q, i, j, k);
+    const long n = g.at(i, j, k);
+    const double ru = X.q[QRU][n], rv = X.q[QRV][n], rw = X.q[QRW][n];
+    const double ro = X.q[QRO][n], tt = X.q[QTT][n];
+    const double L = best, s = C.sff * acc[2] / dt;
+    double dv[NF] = {0.0, 0.0, 0.0, 0.0, 0.0}, dre = 0.0;
+    if (t == 0) {
+        // L = dd / sqrt(fast), fast = uu + gT + 2 sqrt(uu gT), uu = |m|^2 / ro^2
+        const double m2 = ru * ru + rv * rv + rw * rw;
+        const double uu = m2 / (ro * ro), gT = C.gamma * tt;
+        const double sq = sqrt(uu * gT);
+        const double fast = uu + gT + 2.0 * sq;
+        const double dLdf = -0.5 * L / fast;
+        const double dfdu = uu > 0.0 ? 1.0 + gT / sq : 0.0;
+        const double dfdT = C.gamma * (1.0 + (uu > 0.0 ? uu / sq : 0.0));
+        dv[QRU] = dLdf * dfdu * 2.0 * ru / (ro * ro);
+        dv[QRV] = dLdf * dfdu * 2.0 * rv / (ro * ro);
+        dv[QRW] = dLdf * dfdu * 2.0 * rw / (ro * ro);
+        dv[QRO] = dLdf * dfdu * (-2.0 * uu / ro);
+        dv[QTT] = dLdf * dfdT;
+    } else {
+        // thermal 0.5 dd^2 Re Pr cv ro / rk and viscous 0.375 dd^2 Re ro: L ~ Re ro
+        dv[QRO] = L / ro;
+        dre = L / C.re;
+    }
+    for (int f = 0; f < NF; ++f) LX.q[f][n] += s * dv[f];
+    acc[4] += s * dre;
+}
+
 }  // namespace
 
 struct r3d_adj_ctx {
@@ -327,11 +360,11 @@ struct r3d_adj_ctx {
     int nsteps = 0;
     size_t fe = 0;                  // doubles per field array
     std::vector<double*> ckpt;      // X at steps 0, K, 2K, ...
-    std::vector<double*> seg;       // X at the steps of one segment
-    double *x0 = nullptr, *xfinal = nullptr, *x1b = nullptr, *x2b = nullptr;
-    double *f0 = nullptr, *f1 = nullptr, *f2 = nullptr;
-    double *lx = nullptr, *lz = nullptr, *lf = nullptr;
-    double* acc = nullptr;          // g_ore, g_tk, lam_dt, g_theta
+    std::vector<double*> segx, seg1, seg2;  // X_n, X1, X2 of the steps of one segment
+    std::vector<double> segdt;
+    double *x0 = nullptr, *xfinal = nullptr;
+    double* lam[3] = {nullptr, nullptr, nullptr};  // rotating adjoint buffers
+    double* acc = nullptr;          // g_ore, g_tk, <F, lamF>, g_theta, dt's d/dRe
     double* part = nullptr;         // k_ke block sums
     double* bval = nullptr;
     long* bkey = nullptr;
@@ -380,11 +413,11 @@ int blocks_for(const Geo& g) {
     return (int)std::min<long>(ADJ_BLOCKS, (total + AT - 1) / AT);
 }
 
-int objective(r3d_adj_ctx* a, const double* x, bool grad, double* J) {
+int objective(r3d_adj_ctx* a, const double* x, double* lx, double* J) {
     r3d_hip_ctx* w = a->work;
     const int nb = blocks_for(w->g);
     hipLaunchKernelGGL(k_ke, dim3(nb), dim3(AT), 0, w->stream, w->g, state_of(a, (double*)x),
-                       w->M, state_of(a, a->lx), grad ? 1 : 0, a->part);
+                       w->M, state_of(a, lx ? lx : a->lam[0]), lx ? 1 : 0, a->part);
     HIPCHK(hipGetLastError());
     std::vector<double> h(nb);
     HIPCHK(hipMemcpyAsync(h.data(), a->part, nb * sizeof(double), hipMemcpyDeviceToHost,
@@ -396,116 +429,36 @@ int objective(r3d_adj_ctx* a, const double* x, bool grad, double* J) {
     return R3D_OK;
 }
 
-// One truncated forward step from x (1 or 2 stages) into out; returns dt.
-int stage_state(r3d_adj_ctx* a, const double* x, int nstages, double* out, double* dt) {
-    int rc;
-    if ((rc = put_state(a, x))) return rc;
-    if ((rc = r3d_hip_step(a->work, 1, nstages))) return rc;
-    if ((rc = get_state(a, out))) return rc;
-    r3d_hip_scalars s{};
-    if ((rc = r3d_hip_get_scalars(a->work, &s))) return rc;
-    *dt = s.dt;
-    return R3D_OK;
-}
-
-int flux(r3d_adj_ctx* a, const double* x, double* f) {
+// LO = LB + (dF/dX)^T (ca LA + cb LB) at the state x (LO zeroed first).
+int vjp(r3d_adj_ctx* a, const double* x, const double* la, double ca, const double* lb, double cb,
+        double* lo) {
     r3d_hip_ctx* w = a->work;
-    hipLaunchKernelGGL(k_flux_eval, dim3(blocks_for(w->g)), dim3(AT), 0, w->stream, w->g,
-                       state_of(a, (double*)x), w->C, w->M, state_of(a, f));
-    HIPCHK(hipGetLastError());
-    return R3D_OK;
-}
-
-int vjp(r3d_adj_ctx* a, const double* x) {
-    r3d_hip_ctx* w = a->work;
+    HIPCHK(hipMemsetAsync(lo, 0, a->fe * NF * sizeof(double), w->stream));
     hipLaunchKernelGGL(k_flux_vjp, dim3(blocks_for(w->g)), dim3(AT), 0, w->stream, w->g,
-                       state_of(a, (double*)x), w->C, w->M, state_of(a, a->lf),
-                       state_of(a, a->lx), a->acc);
+                       state_of(a, (double*)x), w->C, w->M, state_of(a, (double*)(la ? la : lb)),
+                       la ? ca : 0.0, state_of(a, (double*)lb), cb, state_of(a, lo), a->acc);
     HIPCHK(hipGetLastError());
     return R3D_OK;
 }
 
-int bcon_adj(r3d_adj_ctx* a, const double* x) {
+int bcon_adj(r3d_adj_ctx* a, const double* x, double* lx) {
     r3d_hip_ctx* w = a->work;
     const int nb = (w->g.npx * w->g.npy + AT - 1) / AT;
     hipLaunchKernelGGL(k_bcon_adj, dim3(nb, 2), dim3(AT), 0, w->stream, w->g,
-                       state_of(a, (double*)x), w->C, w->M, state_of(a, a->lx), a->acc);
+                       state_of(a, (double*)x), w->C, w->M, state_of(a, lx), a->acc);
     HIPCHK(hipGetLastError());
     return R3D_OK;
 }
 
-int lin(r3d_adj_ctx* a, const double* F, double coef, double dmul, int src_z, int accumulate,
-        int copy_z) {
+int dt_adjoint(r3d_adj_ctx* a, const double* x, double dt, double* lx) {
     r3d_hip_ctx* w = a->work;
-    hipLaunchKernelGGL(k_lin, dim3(blocks_for(w->g)), dim3(AT), 0, w->stream, w->g,
-                       state_of(a, a->lx), state_of(a, a->lz), state_of(a, a->lf),
-                       state_of(a, (double*)F), coef, dmul, src_z, accumulate, copy_z, a->acc);
+    const int nb = blocks_for(w->g);
+    hipLaunchKernelGGL(k_dt_argmin, dim3(nb), dim3(AT), 0, w->stream, w->g,
+                       state_of(a, (double*)x), w->C, w->M, a->bval, a->bkey);
     HIPCHK(hipGetLastError());
-    return R3D_OK;
-}
-
-// lam_dt * d(dt)/dX at the cell and limit attaining the minimum of the dt
-// limits of x; returns d(dt)/dRe * lam_dt in *gre.
-int dt_adjoint(r3d_adj_ctx* a, const double* x, double lam_dt, double* gre) {
-    r3d_hip_ctx* w = a->work;
-    const Geo& g = w->g;
-    const int nb = blocks_for(g);
-    hipLaunchKernelGGL(k_dt_argmin, dim3(nb), dim3(AT), 0, w->stream, g, state_of(a, (double*)x),
-                       w->C, w->M, a->bval, a->bkey);
+    hipLaunchKernelGGL(k_dt_apply, dim3(1), dim3(64), 0, w->stream, w->g, state_of(a, (double*)x),
+                       w->C, a->bval, a->bkey, nb, dt, state_of(a, lx), a->acc);
     HIPCHK(hipGetLastError());
-    std::vector<double> bv(nb);
-    std::vector<long> bk(nb);
-    HIPCHK(hipMemcpyAsync(bv.data(), a->bval, nb * sizeof(double), hipMemcpyDeviceToHost,
-                          w->stream));
-    HIPCHK(hipMemcpyAsync(bk.data(), a->bkey, nb * sizeof(long), hipMemcpyDeviceToHost,
-                          w->stream));
-    HIPCHK(hipStreamSynchronize(w->stream));
-    double best = 1e300;
-    long key = LONG_MAX;
-    for (int b = 0; b < nb; ++b)
-        if (bv[b] < best || (bv[b] == best && bk[b] < key)) {
-            best = bv[b];
-            key = bk[b];
-        }
-    const long q = key / 4;
-    const int t = (int)(key % 4);
-    const int i = 1 + (int)(q % g.npx), j = 1 + (int)((q / g.npx) % g.npy);
-    const int k = 1 + (int)(q / g.npx / g.npy);
-    const long n = g.at(i, j, k);
-    double v[NF], dv[NF] = {0, 0, 0, 0, 0};
-    for (int f = 0; f < NF; ++f)
-        HIPCHK(hipMemcpy(&v[f], x + f * a->fe + n, sizeof(double), hipMemcpyDeviceToHost));
-    const double ru = v[QRU], rv = v[QRV], rw = v[QRW], ro = v[QRO], tt = v[QTT];
-    const Consts& C = w->C;
-    const double L = best, s = C.sff * lam_dt;
-    double dre = 0.0;
-    if (t == 1) {
-        // L = dd / sqrt(fast), fast = uu + gT + 2 sqrt(uu gT), uu = |m|^2 / ro^2
-        const double m2 = ru * ru + rv * rv + rw * rw;
-        const double uu = m2 / (ro * ro), gT = C.gamma * tt;
-        const double fast = uu + gT + 2.0 * sqrt(uu * gT);
-        const double dLdf = -0.5 * L / fast;
-        const double sq = sqrt(uu * gT);
-        const double dfdu = uu > 0.0 ? 1.0 + gT / sq : 0.0;  // u = 0: velocity derivative 0
-        const double dfdT = C.gamma * (1.0 + (uu > 0.0 ? uu / sq : 0.0));
-        dv[QRU] = dLdf * dfdu * 2.0 * ru / (ro * ro);
-        dv[QRV] = dLdf * dfdu * 2.0 * rv / (ro * ro);
-        dv[QRW] = dLdf * dfdu * 2.0 * rw / (ro * ro);
-        dv[QRO] = dLdf * dfdu * (-2.0 * uu / ro);
-        dv[QTT] = dLdf * dfdT;
-    } else {
-        // thermal 0.5 dd^2 Re Pr cv ro / rk, viscous 0.375 dd^2 Re ro: both L ~ Re ro
-        dv[QRO] = L / ro;
-        dre = L / C.re;
-    }
-    for (int f = 0; f < NF; ++f) {
-        double cur = 0.0;
-        double* p = a->lx + f * a->fe + n;
-        HIPCHK(hipMemcpy(&cur, p, sizeof(double), hipMemcpyDeviceToHost));
-        cur += s * dv[f];
-        HIPCHK(hipMemcpy(p, &cur, sizeof(double), hipMemcpyHostToDevice));
-    }
-    *gre = s * dre;
     return R3D_OK;
 }
 
@@ -531,21 +484,22 @@ int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** o
         r3d_adj_destroy(a);
         return code;
     };
-    double** bufs[] = {&a->x0, &a->xfinal, &a->x1b, &a->x2b, &a->f0, &a->f1, &a->f2,
-                       &a->lx, &a->lz, &a->lf};
+    double** bufs[] = {&a->x0, &a->xfinal, &a->lam[0], &a->lam[1], &a->lam[2]};
     for (double** b : bufs)
         if ((rc = alloc_state(a, b))) return fail(rc);
-    if (hipMalloc((void**)&a->acc, 4 * sizeof(double)) != hipSuccess ||
+    if (hipMalloc((void**)&a->acc, 5 * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&a->part, ADJ_BLOCKS * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&a->bval, ADJ_BLOCKS * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&a->bkey, ADJ_BLOCKS * sizeof(long)) != hipSuccess)
         return fail(set_err(R3D_ENOMEM, "adjoint: small buffers"));
-    a->seg.assign(a->K, nullptr);
-    for (auto& s : a->seg)
-        if ((rc = alloc_state(a, &s))) return fail(rc);
-    // zero everything once: ghost entries of the adjoint fields stay zero
-    double* all[] = {a->x0, a->xfinal, a->x1b, a->x2b, a->f0, a->f1, a->f2, a->lx, a->lz, a->lf};
-    for (double* p : all)
+    for (auto* v : {&a->segx, &a->seg1, &a->seg2}) {
+        v->assign(a->K, nullptr);
+        for (auto& p : *v)
+            if ((rc = alloc_state(a, &p))) return fail(rc);
+    }
+    a->segdt.assign(a->K, 0.0);
+    // ghost entries of the initial state and the adjoint fields stay zero
+    for (double* p : {a->x0, a->xfinal, a->lam[0], a->lam[1], a->lam[2]})
         if (hipMemset(p, 0, a->fe * NF * sizeof(double)) != hipSuccess)
             return fail(set_err(R3D_EHIP, "adjoint: memset"));
     *out = a;
@@ -555,15 +509,13 @@ int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** o
 int r3d_adj_destroy(r3d_adj_ctx* a) {
     if (!a) return R3D_OK;
     if (a->work) (void)hipStreamSynchronize(a->work->stream);
-    double* all[] = {a->x0, a->xfinal, a->x1b, a->x2b, a->f0, a->f1, a->f2, a->lx, a->lz, a->lf,
-                     a->acc, a->part, a->bval};
-    for (double* p : all)
+    for (double* p : {a->x0, a->xfinal, a->lam[0], a->lam[1], a->lam[2], a->acc, a->part,
+                      a->bval})
         if (p) (void)hipFree(p);
     if (a->bkey) (void)hipFree(a->bkey);
-    for (double* p : a->ckpt)
-        if (p) (void)hipFree(p);
-    for (double* p : a->seg)
-        if (p) (void)hipFree(p);
+    for (auto* v : {&a->ckpt, &a->segx, &a->seg1, &a->seg2})
+        for (double* p : *v)
+            if (p) (void)hipFree(p);
     if (a->work) r3d_hip_destroy(a->work);
     delete a;
     return R3D_OK;
@@ -610,7 +562,7 @@ int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J) {
         if ((rc = r3d_hip_step(w, 1, 3))) return rc;
     }
     if ((rc = get_state(a, a->xfinal))) return rc;
-    if ((rc = objective(a, a->xfinal, false, &a->J))) return rc;
+    if ((rc = objective(a, a->xfinal, nullptr, &a->J))) return rc;
     a->nsteps = nsteps;
     if (J) *J = a->J;
     a->fwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
@@ -625,70 +577,62 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
     const Consts& C = w->C;
     const auto t0 = std::chrono::steady_clock::now();
     int rc;
+    double* L = a->lam[0];
+    double* Mb = a->lam[1];
+    double* Nb = a->lam[2];
     double Jdummy;
-    if ((rc = objective(a, a->xfinal, true, &Jdummy))) return rc;  // LX = dJ/dX_N
-    double g_re = 0.0, g_th = 0.0;
+    if ((rc = objective(a, a->xfinal, L, &Jdummy))) return rc;  // L = dJ/dX_N
+    HIPCHK(hipMemsetAsync(a->acc, 0, 5 * sizeof(double), w->stream));
+    const double g1 = C.gam[0], g2 = C.gam[1], g3 = C.gam[2];
+    const double z1 = C.zeta[0], z2 = C.zeta[1];
     const int N = a->nsteps, nseg = (N + a->K - 1) / a->K;
     for (int sgm = nseg - 1; sgm >= 0; --sgm) {
         const int n0 = sgm * a->K, n1 = std::min(N, n0 + a->K);
-        // states X_n0 .. X_{n1-1} of this segment
-        HIPCHK(hipMemcpyAsync(a->seg[0], a->ckpt[sgm], a->fe * NF * sizeof(double),
-                              hipMemcpyDeviceToDevice, w->stream));
+        // X_n, X1, X2 and dt of every step of this segment, from its checkpoint
         if ((rc = put_state(a, a->ckpt[sgm]))) return rc;
-        for (int n = n0 + 1; n < n1; ++n) {
-            if ((rc = r3d_hip_step(w, 1, 3))) return rc;
-            if ((rc = get_state(a, a->seg[n - n0]))) return rc;
+        for (int n = n0; n < n1; ++n) {
+            if ((rc = get_state(a, a->segx[n - n0]))) return rc;
+            double* keep[2] = {a->seg1[n - n0], a->seg2[n - n0]};
+            if ((rc = step_one(w, 3, keep))) return rc;
+            r3d_hip_scalars sc{};
+            if ((rc = r3d_hip_get_scalars(w, &sc))) return rc;
+            a->segdt[n - n0] = sc.dt;
         }
         for (int n = n1 - 1; n >= n0; --n) {
-            const double* xn = a->seg[n - n0];
+            const double* xn = a->segx[n - n0];
+            const double* x1b = a->seg1[n - n0];
+            const double* x2b = a->seg2[n - n0];
             const double* xn1 =
-                n + 1 < n1 ? a->seg[n + 1 - n0] : (n1 == N ? a->xfinal : a->ckpt[sgm + 1]);
-            double dt = 0.0, dt2 = 0.0;
-            if ((rc = stage_state(a, xn, 1, a->x1b, &dt))) return rc;
-            if ((rc = stage_state(a, xn, 2, a->x2b, &dt2))) return rc;
-            if ((rc = flux(a, xn, a->f0)) || (rc = flux(a, a->x1b, a->f1)) ||
-                (rc = flux(a, a->x2b, a->f2)))
-                return rc;
-            HIPCHK(hipMemsetAsync(a->acc, 0, 4 * sizeof(double), w->stream));
-            const double g1 = C.gam[0], g2 = C.gam[1], g3 = C.gam[2];
-            const double z1 = C.zeta[0], z2 = C.zeta[1];
-            // stage 3
-            if ((rc = bcon_adj(a, xn1))) return rc;
-            if ((rc = lin(a, a->f2, g3 * dt, g3, 0, 0, 1))) return rc;  // LF = g3 dt LX, LZ = LX
-            if ((rc = vjp(a, a->x2b))) return rc;                       // LX = LZ + J^T LF
-            if ((rc = lin(a, a->f1, z2 * dt, z2, 1, 0, 0))) return rc;  // LF = z2 dt LZ
-            // stage 2
-            if ((rc = bcon_adj(a, a->x2b))) return rc;
-            if ((rc = lin(a, a->f1, g2 * dt, g2, 0, 1, 1))) return rc;  // LF += g2 dt LX, LZ = LX
-            if ((rc = vjp(a, a->x1b))) return rc;
-            if ((rc = lin(a, a->f0, z1 * dt, z1, 1, 0, 0))) return rc;  // LF = z1 dt LZ
-            // stage 1
-            if ((rc = bcon_adj(a, a->x1b))) return rc;
-            if ((rc = lin(a, a->f0, g1 * dt, g1, 0, 1, 0))) return rc;  // LF += g1 dt LX
-            if ((rc = vjp(a, xn))) return rc;
-            double acc[4];
-            HIPCHK(hipMemcpyAsync(acc, a->acc, sizeof acc, hipMemcpyDeviceToHost, w->stream));
-            HIPCHK(hipStreamSynchronize(w->stream));
-            // ore = 1/Re, tk = ocv/(Re Pr)
-            g_re += acc[0] * (-C.ore * C.ore) + acc[1] * (-C.tk / C.re);
-            g_th += acc[3];
-            double gdt = 0.0;
-            if ((rc = dt_adjoint(a, xn, acc[2], &gdt))) return rc;
-            g_re += gdt;
-            (void)dt2;
+                n + 1 < n1 ? a->segx[n + 1 - n0] : (n1 == N ? a->xfinal : a->ckpt[sgm + 1]);
+            const double dt = a->segdt[n - n0];
+            HIPCHK(hipMemsetAsync(a->acc + 2, 0, sizeof(double), w->stream));
+            // stage 3: L = lam X_{n+1} -> B^T; lamF = g3 dt L; lamZ2 = L
+            if ((rc = bcon_adj(a, xn1, L))) return rc;
+            if ((rc = vjp(a, x2b, nullptr, 0.0, L, g3 * dt, Mb))) return rc;
+            // stage 2: lamF = z2 dt lamZ2 + g2 dt lamX2; lamZ1 = lamX2
+            if ((rc = bcon_adj(a, x2b, Mb))) return rc;
+            if ((rc = vjp(a, x1b, L, z2 * dt, Mb, g2 * dt, Nb))) return rc;
+            // stage 1: lamF = z1 dt lamZ1 + g1 dt lamX1; Z0 = X_n
+            if ((rc = bcon_adj(a, x1b, Nb))) return rc;
+            if ((rc = vjp(a, xn, Mb, z1 * dt, Nb, g1 * dt, L))) return rc;
+            if ((rc = dt_adjoint(a, xn, dt, L))) return rc;
         }
     }
+    double acc[5];
+    HIPCHK(hipMemcpyAsync(acc, a->acc, sizeof acc, hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
+    // ore = 1/Re, tk = ocv/(Re Pr); acc[4] holds the dt limits' d/dRe
+    const double g_re = acc[0] * (-C.ore * C.ore) + acc[1] * (-C.tk / C.re) + acc[4];
     if (grad) {
         const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
         for (int f = 0; f < NF; ++f)
             if (grad[f])
-                HIPCHK(hipMemcpy2D(grad[f], W * sizeof(double), a->lx + f * a->fe + XOFF,
+                HIPCHK(hipMemcpy2D(grad[f], W * sizeof(double), L + f * a->fe + XOFF,
                                    w->g.px * sizeof(double), W * sizeof(double), rows,
                                    hipMemcpyDeviceToHost));
     }
     if (dJ_dre) *dJ_dre = g_re;
-    if (dJ_dth) *dJ_dth = g_th;
+    if (dJ_dth) *dJ_dth = acc[3];
     a->bwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
                     .count();
     return R3D_OK;

@@ -72,6 +72,10 @@ int dev_alloc(r3d_hip_ctx* c, double** p, size_t n);
 int fill_ghosts(r3d_hip_ctx* c);
 // All virtual ranks and the metrics uploaded.
 int check_ready(r3d_hip_ctx* c);
+// One step of nstages stages (the body of r3d_hip_step).  keep[s] (s = 0, 1),
+// if given, receives the state after stage s+1 and its boundary pass (nf
+// fields at a stride of field_elems).
+int step_one(r3d_hip_ctx* c, int nstages, double* const* keep);
 // The FMA-contracted stage kernel (r3d_fma.hip); ty must be the run's tile rows.
 hipError_t launch_stage_fma(const StageArgs& a, const Consts& C, const Metrics& M, int ty,
                             bool mag, unsigned grid, hipStream_t stream);

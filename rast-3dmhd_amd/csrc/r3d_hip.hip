@@ -274,6 +274,36 @@ int r3d::check_ready(r3d_hip_ctx* c) {
     return R3D_OK;
 }
 
+int r3d::step_one(r3d_hip_ctx* c, int nstages, double* const* keep) {
+    int rc;
+    if (c->cfg.tc != 0.0) {
+        // time-dependent plume temperature (src/bcon.cpp:45) from the host libm
+        DevScal d{};
+        HIPCHK(hipMemcpyAsync(&d, c->scal, sizeof d, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const double timt = d.timc + d.timi;
+        c->C.tpr = c->cfg.tp + (1.0 - c->cfg.tp) * exp(-timt / c->cfg.tc);
+    }
+    if (!c->next_valid && (rc = launch_dt(c))) return rc;
+    const size_t fe = field_elems(c->g);
+    for (int s = 0; s < nstages; ++s) {
+        const int last = s == nstages - 1;
+        const int zmode = !last ? 0 : (s == 0 ? 1 : 2);
+        if ((rc = launch_stage(c, s, zmode, nstages == 3 && s == 2))) return rc;
+        if (keep && s < 2 && keep[s])
+            for (int f = 0; f < c->nf; ++f)
+                HIPCHK(hipMemcpyAsync(keep[s] + f * fe, c->xs[c->cur][f], fe * sizeof(double),
+                                      hipMemcpyDeviceToDevice, c->stream));
+    }
+    rc = timed(c, 3, [&] {
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, c->stream, c->scal, c->partial,
+                           c->nblocks + 2 * c->nbb, c->C, nstages == 3 ? 1 : 0, 1);
+    });
+    if (rc) return rc;
+    c->next_valid = nstages == 3;
+    return R3D_OK;
+}
+
 extern "C" {
 
 const char* r3d_hip_last_error(void) { return g_err.c_str(); }
@@ -568,28 +598,8 @@ int r3d_hip_step(r3d_hip_ctx* c, int nsteps, int nstages) {
         return set_err(R3D_EINVAL, "a truncated step (nstages < 3) advances exactly one step");
     int rc = check_ready(c);
     if (rc) return rc;
-    for (int st = 0; st < nsteps; ++st) {
-        if (c->cfg.tc != 0.0) {
-            // time-dependent plume temperature (src/bcon.cpp:45) from the host libm
-            DevScal d{};
-            HIPCHK(hipMemcpyAsync(&d, c->scal, sizeof d, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-            const double timt = d.timc + d.timi;
-            c->C.tpr = c->cfg.tp + (1.0 - c->cfg.tp) * exp(-timt / c->cfg.tc);
-        }
-        if (!c->next_valid && (rc = launch_dt(c))) return rc;
-        for (int s = 0; s < nstages; ++s) {
-            const int last = s == nstages - 1;
-            const int zmode = !last ? 0 : (s == 0 ? 1 : 2);
-            if ((rc = launch_stage(c, s, zmode, nstages == 3 && s == 2))) return rc;
-        }
-        rc = timed(c, 3, [&] {
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, c->stream, c->scal, c->partial,
-                               c->nblocks + 2 * c->nbb, c->C, nstages == 3 ? 1 : 0, 1);
-        });
-        if (rc) return rc;
-        c->next_valid = nstages == 3;
-    }
+    for (int st = 0; st < nsteps; ++st)
+        if ((rc = step_one(c, nstages, nullptr))) return rc;
     if (c->timing) return drain_timing(c);
     return R3D_OK;
 }
