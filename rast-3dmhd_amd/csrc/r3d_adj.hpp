@@ -50,7 +50,19 @@ struct Stencil {
 // Metrics of the cell's column and plane, and which wall it is on.
 struct CellMet {
     double dxx, d2xx, dyy, d2yy, dzz, d2zz, rk, dk;
-    int wall;  // 0 interior, 1 top (k = 1), 2 bottom (k = npz)
+    int wall;     // 0 interior, 1 top (k = 1), 2 bottom (k = npz)
+    int mag_bot;  // LMAG one-sided stencils on this bottom-wall cell (npez > 1)
+};
+
+// Magnetic field at the stencil points, and the wall extras of the LMAG
+// one-sided stencils: RO two planes in (for WW there), BX, BY two and three
+// planes in (k+2, k+3 on the top wall, k-2, k-3 on the bottom wall).
+struct BPt {
+    double bx, by, bz;
+};
+struct MagStencil {
+    BPt p[NPTS];
+    double ro2, bx2, by2, bx3, by3;
 };
 
 // F = (FU, FV, FW, FR, FT) in the order of the primaries QRU..QTT.
@@ -378,6 +390,267 @@ __host__ __device__ inline void cell_flux_vjp(const Stencil& s, const CellMet& m
         AP[q].tt += P[q].ro * ap[q];
     }
     AP[P_C].ro -= ri[P_C] * ri[P_C] * ari;
+}
+
+// F of an MHD cell: the hydrodynamic F followed by the LMAG terms of the
+// Fortran FLUXES (golden/fortran/3dmhdsub.f:2146-2651) in the stage kernel's
+// operation order; F = (FU, FV, FW, FR, FT, WW1, WW2, WW3).
+__host__ __device__ inline void cell_flux_mag(const Stencil& s, const MagStencil& b,
+                                              const CellMet& m, const Consts& C, double F[8]) {
+    cell_flux(s, m, C, F);
+    double u[NPTS], v[NPTS], w[NPTS], bx[NPTS], by[NPTS], bz[NPTS];
+    for (int q = 0; q < NPTS; ++q) {
+        const double ri = 1.0 / s.p[q].ro;
+        u[q] = s.p[q].ru * ri;
+        v[q] = s.p[q].rv * ri;
+        w[q] = s.p[q].rw * ri;
+        bx[q] = b.p[q].bx;
+        by[q] = b.p[q].by;
+        bz[q] = b.p[q].bz;
+    }
+    const double R = 1.0 / s.p[P_C].ro;
+    const double hxd = C.hx, dxx = m.dxx, d2xx = m.d2xx, dyy = m.dyy, d2yy = m.d2yy;
+    const double dzz = m.dzz, d2zz = m.d2zz;
+#define DXS(q) ((q[P_E] - q[P_W]) * hxd * dxx)
+#define DYS(q) ((q[P_N] - q[P_S]) * C.hy * dyy)
+#define DZC(q) ((q[P_U] - q[P_D]) * C.hz * dzz)
+#define LAPX(q, c2) ((q[P_E] - q[P_W]) * hxd * d2xx + (q[P_E] - c2 + q[P_W]) * C.h2x * dxx * dxx)
+#define LAPY(q, c2) ((q[P_N] - q[P_S]) * C.hy * d2yy + (q[P_N] - c2 + q[P_S]) * C.h2y * dyy * dyy)
+#define LAPZ(q, c2) ((q[P_U] - q[P_D]) * C.hz * d2zz + (q[P_U] - c2 + q[P_D]) * C.h2z * dzz * dzz)
+    {
+        const double bxy = DYS(bx), byx = DXS(by), bzx = DXS(bz), bxz = DZC(bx);
+        double fu = F[QRU];
+        fu = fu + C.obeta * by[P_C] * (bxy - byx) - C.obeta * bz[P_C] * bzx;
+        fu = fu + C.obeta * bz[P_C] * bxz;
+        F[QRU] = fu;
+    }
+    {
+        const double byx = DXS(by), bxy = DYS(bx), byz = DZC(by), bzy = DYS(bz);
+        double fv = F[QRV];
+        fv = fv + C.obeta * bx[P_C] * (byx - bxy);
+        fv = fv + C.obeta * bz[P_C] * (byz - bzy);
+        F[QRV] = fv;
+    }
+    {
+        const double bzx = DXS(bz), bxz = DZC(bx), byz = DZC(by), bzy = DYS(bz);
+        double fw = F[QRW];
+        fw = fw + C.obeta * bx[P_C] * bzx;
+        fw = fw - C.obeta * bx[P_C] * bxz;
+        fw = fw - C.obeta * by[P_C] * (byz - bzy);
+        F[QRW] = fw;
+    }
+    {
+        const double bxy = DYS(bx), byx = DXS(by), bzx = DXS(bz), bxz = DZC(bx);
+        const double byz = DZC(by), bzy = DYS(bz);
+        double ft = F[QTT];
+        ft = ft + (bxy * bxy + byx * byx + bzx * bzx - 2.0 * byx * bxy) * R * C.ohm;
+        ft = ft + (bxz * bxz - 2.0 * bxz * bzx) * R * C.ohm;
+        ft = ft + (byz * byz + bzy * bzy - 2.0 * byz * bzy) * R * C.ohm;
+        F[QTT] = ft;
+    }
+    const bool top = m.wall == 1, bot = m.wall == 2 && m.mag_bot;
+    const double w2 = (top || bot) ? s.rw2 * (1.0 / b.ro2) : 0.0;
+    double wzm = DZC(w);
+    if (top) wzm = (4.0 * w[P_U] - w2) * C.hz * dzz;
+    if (bot) wzm = (w2 - 4.0 * w[P_D]) * C.hz * dzz;
+    {
+        const double b2c = 2.0 * bx[P_C];
+        double lbz = LAPZ(bx, b2c);
+        if (top)
+            lbz = (-3.0 * bx[P_C] + 4.0 * bx[P_U] - b.bx2) * C.hz * d2zz +
+                  (2.0 * bx[P_C] - 5.0 * bx[P_U] + 4.0 * b.bx2 - b.bx3) * C.h2z * dzz * dzz;
+        if (bot)
+            lbz = (3.0 * bx[P_C] - 4.0 * bx[P_D] + b.bx2) * C.hz * d2zz +
+                  (2.0 * bx[P_C] - 5.0 * bx[P_D] + 4.0 * b.bx2 - b.bx3) * C.h2z * dzz * dzz;
+        double w1 = by[P_C] * DYS(u) - bx[P_C] * DYS(v);
+        w1 = w1 + bz[P_C] * DZC(u) - bx[P_C] * wzm;
+        w1 = w1 - u[P_C] * DXS(bx) - v[P_C] * DYS(bx);
+        w1 = w1 + C.orm * (LAPX(bx, b2c) + LAPY(bx, b2c));
+        w1 = w1 - w[P_C] * DZC(bx);
+        w1 = w1 + C.orm * lbz;
+        F[QBX] = w1;
+    }
+    {
+        const double b2c = 2.0 * by[P_C];
+        double lbz = LAPZ(by, b2c);
+        if (top)
+            lbz = (-3.0 * by[P_C] + 4.0 * by[P_U] - b.by2) * C.hz * d2zz +
+                  (2.0 * by[P_C] - 5.0 * by[P_U] + 4.0 * b.by2 - b.by3) * C.h2z * dzz * dzz;
+        if (bot)
+            lbz = (3.0 * by[P_C] - 4.0 * by[P_D] + b.by2) * C.hz * d2zz +
+                  (2.0 * by[P_C] - 5.0 * by[P_D] + 4.0 * b.by2 - b.by3) * C.h2z * dzz * dzz;
+        double w2f = bx[P_C] * DXS(v) - by[P_C] * DXS(u);
+        w2f = w2f + bz[P_C] * DZC(v) - by[P_C] * wzm;
+        w2f = w2f - u[P_C] * DXS(by);
+        w2f = w2f + C.orm * LAPX(by, b2c);
+        w2f = w2f - v[P_C] * DYS(by);
+        w2f = w2f + C.orm * LAPY(by, b2c);
+        w2f = w2f - w[P_C] * DZC(by);
+        w2f = w2f + C.orm * lbz;
+        F[QBY] = w2f;
+    }
+    {
+        const double b2c = 2.0 * bz[P_C];
+        double w3 = bx[P_C] * DXS(w) - bz[P_C] * DXS(u);
+        w3 = w3 + by[P_C] * DYS(w) - bz[P_C] * DYS(v);
+        w3 = w3 - u[P_C] * DXS(bz);
+        w3 = w3 + C.orm * LAPX(bz, b2c);
+        w3 = w3 - v[P_C] * DYS(bz);
+        w3 = w3 + C.orm * LAPY(bz, b2c);
+        w3 = w3 - w[P_C] * DZC(bz);
+        w3 = w3 + C.orm * LAPZ(bz, b2c);
+        F[QBZ] = w3;
+    }
+#undef DXS
+#undef DYS
+#undef DZC
+#undef LAPX
+#undef LAPY
+#undef LAPZ
+}
+
+// Reverse of cell_flux_mag: lam[0..7] weights F; A (hydrodynamic primaries,
+// A.rw2 included) and AB (B at the points and the wall extras) receive the
+// derivatives; g_ore, g_tk as for cell_flux_vjp (the LMAG terms do not
+// depend on Re).
+__host__ __device__ inline void cell_flux_mag_vjp(const Stencil& s, const MagStencil& b,
+                                                  const CellMet& m, const Consts& C,
+                                                  const double lam[8], Stencil& A,
+                                                  MagStencil& AB, double& g_ore, double& g_tk) {
+    cell_flux_vjp(s, m, C, lam, A, g_ore, g_tk);
+    double ri[NPTS], u[NPTS], v[NPTS], w[NPTS], bx[NPTS], by[NPTS], bz[NPTS];
+    double au[NPTS], av[NPTS], aw[NPTS];
+    for (int q = 0; q < NPTS; ++q) {
+        ri[q] = 1.0 / s.p[q].ro;
+        u[q] = s.p[q].ru * ri[q];
+        v[q] = s.p[q].rv * ri[q];
+        w[q] = s.p[q].rw * ri[q];
+        bx[q] = b.p[q].bx;
+        by[q] = b.p[q].by;
+        bz[q] = b.p[q].bz;
+        au[q] = av[q] = aw[q] = 0.0;
+        AB.p[q].bx = AB.p[q].by = AB.p[q].bz = 0.0;
+    }
+    AB.ro2 = AB.bx2 = AB.by2 = AB.bx3 = AB.by3 = 0.0;
+    const double hxd = C.hx * m.dxx, hyd = C.hy * m.dyy, hzd = C.hz * m.dzz;
+    const double lx1 = C.hx * m.d2xx, lx2 = C.h2x * m.dxx * m.dxx;
+    const double ly1 = C.hy * m.d2yy, ly2 = C.h2y * m.dyy * m.dyy;
+    const double lz1 = C.hz * m.d2zz, lz2 = C.h2z * m.dzz * m.dzz;
+    const double LU = lam[QRU], LV = lam[QRV], LW = lam[QRW], LT = lam[QTT];
+    const double L1 = lam[QBX], L2 = lam[QBY], L3 = lam[QBZ];
+    const double ob = C.obeta, oh = C.ohm, R = ri[P_C];
+    const double bx0 = bx[P_C], by0 = by[P_C], bz0 = bz[P_C];
+    const double u0 = u[P_C], v0 = v[P_C], w0 = w[P_C];
+    auto dx = [&](const double* q) { return (q[P_E] - q[P_W]) * hxd; };
+    auto dy = [&](const double* q) { return (q[P_N] - q[P_S]) * hyd; };
+    auto dz = [&](const double* q) { return (q[P_U] - q[P_D]) * hzd; };
+    const double bxy = dy(bx), byx = dx(by), bzx = dx(bz), bxz = dz(bx), byz = dz(by);
+    const double bzy = dy(bz), bxx = dx(bx), byy = dy(by), bzz = dz(bz);
+    const double ux = dx(u), vx = dx(v), wx = dx(w), uy = dy(u), vy = dy(v), wy = dy(w);
+    const double uz = dz(u), vz = dz(v);
+    const bool top = m.wall == 1, bot = m.wall == 2 && m.mag_bot;
+    const double ri2 = (top || bot) ? 1.0 / b.ro2 : 0.0;
+    const double w2 = s.rw2 * ri2;
+    double wzm = dz(w);
+    if (top) wzm = (4.0 * w[P_U] - w2) * hzd;
+    if (bot) wzm = (w2 - 4.0 * w[P_D]) * hzd;
+
+    // adjoints of the first derivatives of B
+    const double a_bxy = LU * ob * by0 - LV * ob * bx0 + LT * R * oh * (2.0 * bxy - 2.0 * byx) - L1 * v0;
+    const double a_byx = -LU * ob * by0 + LV * ob * bx0 + LT * R * oh * (2.0 * byx - 2.0 * bxy) - L2 * u0;
+    const double a_bzx = -LU * ob * bz0 + LW * ob * bx0 + LT * R * oh * (2.0 * bzx - 2.0 * bxz) - L3 * u0;
+    const double a_bxz = LU * ob * bz0 - LW * ob * bx0 + LT * R * oh * (2.0 * bxz - 2.0 * bzx) - L1 * w0;
+    const double a_byz = LV * ob * bz0 - LW * ob * by0 + LT * R * oh * (2.0 * byz - 2.0 * bzy) - L2 * w0;
+    const double a_bzy = -LV * ob * bz0 + LW * ob * by0 + LT * R * oh * (2.0 * bzy - 2.0 * byz) - L3 * v0;
+    const double a_bxx = -L1 * u0, a_byy = -L2 * v0, a_bzz = -L3 * w0;
+    auto add_d = [&](double a, int pp, int pm, double h, double BPt::*f) {
+        AB.p[pp].*f += a * h;
+        AB.p[pm].*f -= a * h;
+    };
+    add_d(a_bxy, P_N, P_S, hyd, &BPt::bx);
+    add_d(a_byx, P_E, P_W, hxd, &BPt::by);
+    add_d(a_bzx, P_E, P_W, hxd, &BPt::bz);
+    add_d(a_bxz, P_U, P_D, hzd, &BPt::bx);
+    add_d(a_byz, P_U, P_D, hzd, &BPt::by);
+    add_d(a_bzy, P_N, P_S, hyd, &BPt::bz);
+    add_d(a_bxx, P_E, P_W, hxd, &BPt::bx);
+    add_d(a_byy, P_N, P_S, hyd, &BPt::by);
+    add_d(a_bzz, P_U, P_D, hzd, &BPt::bz);
+    // resistive Laplacians (orm times each component's full Laplacian)
+    const double Lb[3] = {L1 * C.orm, L2 * C.orm, L3 * C.orm};
+    double BPt::*fld[3] = {&BPt::bx, &BPt::by, &BPt::bz};
+    for (int c = 0; c < 3; ++c) {
+        double BPt::*f = fld[c];
+        const double L = Lb[c];
+        AB.p[P_E].*f += L * (lx1 + lx2);
+        AB.p[P_W].*f += L * (-lx1 + lx2);
+        AB.p[P_N].*f += L * (ly1 + ly2);
+        AB.p[P_S].*f += L * (-ly1 + ly2);
+        AB.p[P_C].*f += L * (-2.0 * lx2 - 2.0 * ly2);
+        if (c < 2 && top) {
+            AB.p[P_C].*f += L * (-3.0 * lz1 + 2.0 * lz2);
+            AB.p[P_U].*f += L * (4.0 * lz1 - 5.0 * lz2);
+            (c == 0 ? AB.bx2 : AB.by2) += L * (-lz1 + 4.0 * lz2);
+            (c == 0 ? AB.bx3 : AB.by3) += L * (-lz2);
+        } else if (c < 2 && bot) {
+            AB.p[P_C].*f += L * (3.0 * lz1 + 2.0 * lz2);
+            AB.p[P_D].*f += L * (-4.0 * lz1 - 5.0 * lz2);
+            (c == 0 ? AB.bx2 : AB.by2) += L * (lz1 + 4.0 * lz2);
+            (c == 0 ? AB.bx3 : AB.by3) += L * (-lz2);
+        } else {
+            AB.p[P_U].*f += L * (lz1 + lz2);
+            AB.p[P_D].*f += L * (-lz1 + lz2);
+            AB.p[P_C].*f += L * (-2.0 * lz2);
+        }
+    }
+    // B at the cell
+    AB.p[P_C].bx += LV * ob * (byx - bxy) + LW * ob * (bzx - bxz) - L1 * vy - L1 * wzm + L2 * vx +
+                    L3 * wx;
+    AB.p[P_C].by += LU * ob * (bxy - byx) - LW * ob * (byz - bzy) + L1 * uy - L2 * ux - L2 * wzm +
+                    L3 * wy;
+    AB.p[P_C].bz += -LU * ob * bzx + LU * ob * bxz + LV * ob * (byz - bzy) + L1 * uz + L2 * vz -
+                    L3 * ux - L3 * vy;
+    // velocities: at the cell and through their derivatives
+    au[P_C] += -L1 * bxx - L2 * byx - L3 * bzx;
+    av[P_C] += -L1 * bxy - L2 * byy - L3 * bzy;
+    aw[P_C] += -L1 * bxz - L2 * byz - L3 * bzz;
+    const double a_uy = L1 * by0, a_vy = -L1 * bx0 - L3 * bz0, a_uz = L1 * bz0;
+    const double a_wzm = -L1 * bx0 - L2 * by0, a_vx = L2 * bx0, a_ux = -L2 * by0 - L3 * bz0;
+    const double a_vz = L2 * bz0, a_wx = L3 * bx0, a_wy = L3 * by0;
+    au[P_N] += a_uy * hyd; au[P_S] -= a_uy * hyd;
+    av[P_N] += a_vy * hyd; av[P_S] -= a_vy * hyd;
+    au[P_U] += a_uz * hzd; au[P_D] -= a_uz * hzd;
+    av[P_U] += a_vz * hzd; av[P_D] -= a_vz * hzd;
+    av[P_E] += a_vx * hxd; av[P_W] -= a_vx * hxd;
+    au[P_E] += a_ux * hxd; au[P_W] -= a_ux * hxd;
+    aw[P_E] += a_wx * hxd; aw[P_W] -= a_wx * hxd;
+    aw[P_N] += a_wy * hyd; aw[P_S] -= a_wy * hyd;
+    double aw2 = 0.0;
+    if (top) {
+        aw[P_U] += 4.0 * a_wzm * hzd;
+        aw2 -= a_wzm * hzd;
+    } else if (bot) {
+        aw2 += a_wzm * hzd;
+        aw[P_D] -= 4.0 * a_wzm * hzd;
+    } else {
+        aw[P_U] += a_wzm * hzd;
+        aw[P_D] -= a_wzm * hzd;
+    }
+    // Ohmic heating's 1/RO at the cell
+    const double q1 = bxy * bxy + byx * byx + bzx * bzx - 2.0 * byx * bxy;
+    const double q2 = bxz * bxz - 2.0 * bxz * bzx;
+    const double q3 = byz * byz + bzy * bzy - 2.0 * byz * bzy;
+    const double aR = LT * oh * (q1 + q2 + q3);
+    // derived quantities back to the primaries
+    for (int q = 0; q < NPTS; ++q) {
+        A.p[q].ru += ri[q] * au[q];
+        A.p[q].rv += ri[q] * av[q];
+        A.p[q].rw += ri[q] * aw[q];
+        A.p[q].ro -= ri[q] * (u[q] * au[q] + v[q] * av[q] + w[q] * aw[q]);
+    }
+    A.p[P_C].ro -= R * R * aR;
+    A.rw2 += ri2 * aw2;
+    AB.ro2 -= w2 * ri2 * aw2;
 }
 
 }  // namespace r3d

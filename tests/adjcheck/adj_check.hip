@@ -12,6 +12,7 @@ namespace {
 
 struct Dims {
     int nx, ny, nz, top, bot;  // top/bot: the rank holds the global top/bottom wall
+    int mag_bot;               // LMAG one-sided stencils on the bottom wall (npez > 1)
 };
 
 long at(const Dims& d, int i, int j, int k) { return ((long)k * d.ny + j) * d.nx + i; }
@@ -25,6 +26,7 @@ void gather(const Dims& d, const double* const* X, int i, int j, int k, Stencil&
         s.p[q] = {X[0][n], X[1][n], X[2][n], X[3][n], X[4][n]};
     }
     m.wall = (d.top && k == 1) ? 1 : ((d.bot && k == d.nz - 2) ? 2 : 0);
+    m.mag_bot = m.wall == 2 && d.mag_bot;
     s.rw2 = m.wall == 1 ? X[2][at(d, i, j, k + 2)] : (m.wall == 2 ? X[2][at(d, i, j, k - 2)] : 0.0);
     m.dxx = met[0][i];
     m.d2xx = met[1][i];
@@ -36,11 +38,33 @@ void gather(const Dims& d, const double* const* X, int i, int j, int k, Stencil&
     m.dk = met[7][k];
 }
 
+// B at the stencil points and the LMAG wall extras (B = bx by bz).
+void gather_mag(const Dims& d, const double* const* X, const double* const* B, int i, int j,
+                int k, const CellMet& m, MagStencil& b) {
+    for (int q = 0; q < NPTS; ++q) {
+        int di, dj, dk;
+        pt_offset(q, di, dj, dk);
+        const long n = at(d, i + di, j + dj, k + dk);
+        b.p[q] = {B[0][n], B[1][n], B[2][n]};
+    }
+    b.ro2 = b.bx2 = b.by2 = b.bx3 = b.by3 = 0.0;
+    const int dir = m.wall == 1 ? 1 : -1;
+    if (m.wall == 1 || m.mag_bot) {
+        const long n2 = at(d, i, j, k + 2 * dir), n3 = at(d, i, j, k + 3 * dir);
+        b.ro2 = X[3][n2];
+        b.bx2 = B[0][n2];
+        b.by2 = B[1][n2];
+        b.bx3 = B[0][n3];
+        b.by3 = B[1][n3];
+    }
+}
+
 }  // namespace
 
 extern "C" {
 
 // consts: c13 c43 ore ocv grav hx h2x hy h2y hz h2z omx omz tk t1 t2 t3 lrot
+//         obeta orm ohm
 static Consts make_consts(const double* k) {
     Consts C{};
     C.c13 = k[0];
@@ -61,6 +85,9 @@ static Consts make_consts(const double* k) {
     C.t2 = k[15];
     C.t3 = k[16];
     C.lrot = (int)k[17];
+    C.obeta = k[18];
+    C.orm = k[19];
+    C.ohm = k[20];
     return C;
 }
 
@@ -68,7 +95,7 @@ static Consts make_consts(const double* k) {
 // met = dxx d2xx dyy d2yy dzz d2zz rkapa dkapa; F = fu fv fw fr ft.
 void adj_flux(const int* dims, const double* const* X, const double* const* met,
               const double* kc, double* const* F) {
-    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4]};
+    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4], dims[5]};
     const Consts C = make_consts(kc);
     for (int k = 1; k <= d.nz - 2; ++k)
         for (int j = 1; j <= d.ny - 2; ++j)
@@ -87,7 +114,7 @@ void adj_flux(const int* dims, const double* const* X, const double* const* met,
 // interior cells; g[0] = d(lamF.F)/d(ore), g[1] = d/d(tk).
 void adj_vjp(const int* dims, const double* const* X, const double* const* met, const double* kc,
              const double* const* lamF, double* const* lamX, double* g) {
-    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4]};
+    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4], dims[5]};
     const Consts C = make_consts(kc);
     g[0] = g[1] = 0.0;
     for (int k = 1; k <= d.nz - 2; ++k)
@@ -113,6 +140,74 @@ void adj_vjp(const int* dims, const double* const* X, const double* const* met, 
                     lamX[4][p] += A.p[q].tt;
                 }
                 if (m.wall) lamX[2][at(d, i, j, m.wall == 1 ? k + 2 : k - 2)] += A.rw2;
+            }
+}
+
+// MHD: X = ru rv rw ro tt, B = bx by bz; F = fu fv fw fr ft ww1 ww2 ww3.
+void adj_flux_mag(const int* dims, const double* const* X, const double* const* B,
+                  const double* const* met, const double* kc, double* const* F) {
+    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4], dims[5]};
+    const Consts C = make_consts(kc);
+    for (int k = 1; k <= d.nz - 2; ++k)
+        for (int j = 1; j <= d.ny - 2; ++j)
+            for (int i = 1; i <= d.nx - 2; ++i) {
+                Stencil s;
+                MagStencil b;
+                CellMet m;
+                gather(d, X, i, j, k, s, m, met);
+                gather_mag(d, X, B, i, j, k, m, b);
+                double f[8];
+                cell_flux_mag(s, b, m, C, f);
+                const long n = at(d, i, j, k);
+                for (int q = 0; q < 8; ++q) F[q][n] = f[q];
+            }
+}
+
+// lamXB = (lamX ru..tt, lamB bx by bz) accumulated over the whole array.
+void adj_vjp_mag(const int* dims, const double* const* X, const double* const* B,
+                 const double* const* met, const double* kc, const double* const* lamF,
+                 double* const* lamX, double* g) {
+    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4], dims[5]};
+    const Consts C = make_consts(kc);
+    g[0] = g[1] = 0.0;
+    for (int k = 1; k <= d.nz - 2; ++k)
+        for (int j = 1; j <= d.ny - 2; ++j)
+            for (int i = 1; i <= d.nx - 2; ++i) {
+                Stencil s, A;
+                MagStencil b, AB;
+                CellMet m;
+                gather(d, X, i, j, k, s, m, met);
+                gather_mag(d, X, B, i, j, k, m, b);
+                const long n = at(d, i, j, k);
+                double lam[8];
+                for (int q = 0; q < 8; ++q) lam[q] = lamF[q][n];
+                double go, gt;
+                cell_flux_mag_vjp(s, b, m, C, lam, A, AB, go, gt);
+                g[0] += go;
+                g[1] += gt;
+                for (int q = 0; q < NPTS; ++q) {
+                    int di, dj, dk;
+                    pt_offset(q, di, dj, dk);
+                    const long p = at(d, i + di, j + dj, k + dk);
+                    lamX[0][p] += A.p[q].ru;
+                    lamX[1][p] += A.p[q].rv;
+                    lamX[2][p] += A.p[q].rw;
+                    lamX[3][p] += A.p[q].ro;
+                    lamX[4][p] += A.p[q].tt;
+                    lamX[5][p] += AB.p[q].bx;
+                    lamX[6][p] += AB.p[q].by;
+                    lamX[7][p] += AB.p[q].bz;
+                }
+                if (m.wall) {
+                    const int dir = m.wall == 1 ? 1 : -1;
+                    const long n2 = at(d, i, j, k + 2 * dir), n3 = at(d, i, j, k + 3 * dir);
+                    lamX[2][n2] += A.rw2;
+                    lamX[3][n2] += AB.ro2;
+                    lamX[5][n2] += AB.bx2;
+                    lamX[6][n2] += AB.by2;
+                    lamX[5][n3] += AB.bx3;
+                    lamX[6][n3] += AB.by3;
+                }
             }
 }
 

@@ -5,7 +5,9 @@ run on the CPU through tests/adjcheck/adj_check.hip:
   reference CPU build) bit for bit, walls and rotation included;
 * cell_flux_vjp is the transpose of the flux Jacobian: for random directions
   the adjoint's inner product matches a central finite difference of
-  lam . F, field by field, and likewise for the coefficients ore and tk.
+  lam . F, field by field, and likewise for the coefficients ore and tk;
+* the same for the MHD pair cell_flux_mag / cell_flux_mag_vjp against the
+  oracle's LMAG branch, with a fully three-dimensional B.
 """
 import ctypes
 import os
@@ -42,6 +44,8 @@ def lib(tmp_path_factory):
     dp = ctypes.POINTER(ctypes.c_double)
     L.adj_flux.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, dp, pp]
     L.adj_vjp.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, dp, pp, pp, dp]
+    L.adj_flux_mag.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, pp, dp, pp]
+    L.adj_vjp_mag.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, pp, dp, pp, pp, dp]
     return L
 
 
@@ -54,8 +58,8 @@ class Rank:
     def __init__(self, sim, r):
         self.sim, self.r = sim, r
         py, pz = sim.rank_coords(r)
-        self.dims = (ctypes.c_int * 5)(sim.nx, sim.ny, sim.nz, int(pz == 0),
-                                       int(pz == sim.npez - 1))
+        self.dims = (ctypes.c_int * 6)(sim.nx, sim.ny, sim.nz, int(pz == 0),
+                                       int(pz == sim.npez - 1), int(sim.npez > 1))
         self.met = [np.ascontiguousarray(sim.metric(r, n)) for n in
                     ("dxxdx", "d2xxdx2", "dyydy", "d2yydy2", "dzzdz", "d2zzdz2", "rkapa", "dkapa")]
         d = sim.derived()
@@ -66,7 +70,8 @@ class Rank:
         d = self.d
         return np.array([d["c13"], d["c43"], ore, d["ocv"], GRAV, d["hx"], d["h2x"], d["hy"],
                          d["h2y"], d["hz"], d["h2z"], d["omx"], d["omz"], tk, d["ocv"] * ore,
-                         d["c13"] * ore, d["c43"] * d["ocv"] * ore, 1.0])
+                         d["c13"] * ore, d["c43"] * d["ocv"] * ore, 1.0, d["obeta"], d["orm"],
+                         d["obeta"] * d["orm"] * d["ocv"]])
 
     def flux(self, L, X, k=None):
         F = [np.zeros_like(X[0]) for _ in range(5)]
@@ -74,6 +79,20 @@ class Rank:
         L.adj_flux(self.dims, _pp(X), _pp(self.met), kk.ctypes.data_as(
             ctypes.POINTER(ctypes.c_double)), _pp(F))
         return F
+
+    def flux_mag(self, L, X, B):
+        F = [np.zeros_like(X[0]) for _ in range(8)]
+        L.adj_flux_mag(self.dims, _pp(X), _pp(B), _pp(self.met),
+                       self.k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _pp(F))
+        return F
+
+    def vjp_mag(self, L, X, B, lam):
+        out = [np.zeros_like(X[0]) for _ in range(8)]
+        g = np.zeros(2)
+        L.adj_vjp_mag(self.dims, _pp(X), _pp(B), _pp(self.met),
+                      self.k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _pp(lam),
+                      _pp(out), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
 
     def vjp(self, L, X, lam):
         out = [np.zeros_like(X[0]) for _ in range(5)]
@@ -153,3 +172,62 @@ def test_vjp_coefficient_derivatives(lib, sim):
             h = (vo - ore) if idx == 0 else (vt - tk)
             fd = (fp - fm) / (2 * h)
             assert abs(g[idx] - fd) <= 1e-7 * abs(fd) + 1e-12, (r, idx, g[idx], fd)
+
+
+MAG_ARGS = ["--npx", "12", "--npy", "10", "--npz", "18", "--npey", "1", "--npez", "2",
+            "--lmag", "1", "--rm", "5", "--beta", "2", "--ampb", "1", "--bfh", "2",
+            "--bzp", "10", "--lrot", "1", "--ry", "0.3", "--ang", "0.4"]
+B_NAMES = ["bx", "by", "bz"]
+
+
+@pytest.fixture(scope="module")
+def msim():
+    """An MHD state with a fully three-dimensional B (the layer alone has
+    by = bz = 0)."""
+    s = oracle.Sim(MAG_ARGS)
+    s.step(2)
+    rng = np.random.default_rng(9)
+    for r in range(s.nranks):
+        for n in B_NAMES:
+            f = s.field(r, n)
+            f[1:-1, 1:-1, 1:-1] += 0.2 * rng.uniform(-1, 1, f[1:-1, 1:-1, 1:-1].shape)
+    s.communicate()
+    return s
+
+
+def test_cell_flux_mag_reproduces_oracle_fluxes(lib, msim):
+    for r in range(msim.nranks):
+        rk = Rank(msim, r)
+        X = [msim.field(r, n).copy() for n in X_NAMES]
+        B = [msim.field(r, n).copy() for n in B_NAMES]
+        F = rk.flux_mag(lib, X, B)
+        msim.fluxes(r)
+        for f, name in zip(F, F_NAMES + ["ww1", "ww2", "ww3"]):
+            want = msim.field(r, name)[1:-1, 1:-1, 1:-1]
+            got = f[1:-1, 1:-1, 1:-1]
+            assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), (r, name)
+        for q, n in enumerate(X_NAMES):
+            msim.field(r, n)[...] = X[q]
+
+
+@pytest.mark.parametrize("field", range(8), ids=X_NAMES + B_NAMES)
+def test_vjp_mag_matches_finite_differences(lib, msim, field):
+    rng = np.random.default_rng(31 + field)
+    names = X_NAMES + B_NAMES
+    for r in range(msim.nranks):
+        rk = Rank(msim, r)
+        XB = [msim.field(r, n).copy() for n in names]
+        mask = _interior_mask(XB[0].shape)
+        lam = [rng.uniform(-1, 1, XB[0].shape) * mask for _ in range(8)]
+        dirn = rng.uniform(-1, 1, XB[0].shape) * np.abs(XB[field]).max()
+        lamX = rk.vjp_mag(lib, XB[:5], XB[5:], lam)
+        adj = float(np.sum(lamX[field] * dirn))
+        # lam . F is quadratic in the momenta and in B; 1/RO curves it in RO
+        eps = {3: 1e-6, 4: 1e-5}.get(field, 1e-4)
+        P = [x.copy() for x in XB]
+        M = [x.copy() for x in XB]
+        P[field] += eps * dirn
+        M[field] -= eps * dirn
+        phi = lambda F: sum(float(np.sum(l * f)) for l, f in zip(lam, F))
+        fd = (phi(rk.flux_mag(lib, P[:5], P[5:])) - phi(rk.flux_mag(lib, M[:5], M[5:]))) / (2 * eps)
+        assert abs(adj - fd) <= 1e-7 * abs(fd) + 1e-12, (r, names[field], adj, fd)
