@@ -16,7 +16,8 @@
  * backward pass recomputes the forward states of one segment at a time from
  * its checkpoint, then the two intermediate stage states of each step.
  *
- * Hydrodynamic runs only (lmag = 0) and tc = 0.  Errors follow r3d_hip.h.
+ * Hydrodynamic and MHD runs (with LMAG the state and the gradient have the
+ * eight primaries R3D_RU..R3D_BZ); tc = 0.  Errors follow r3d_hip.h.
  */
 #ifndef R3D_ADJ_H
 #define R3D_ADJ_H
@@ -32,12 +33,14 @@ typedef struct r3d_adj_ctx r3d_adj_ctx;
 int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** out);
 int r3d_adj_destroy(r3d_adj_ctx* a);
 int r3d_adj_set_metrics(r3d_adj_ctx* a, const r3d_hip_metrics* m);
-/* X0: fields[R3D_RU..R3D_TT], each (npx+2)(npy+2)(npz+2) doubles, x fastest. */
+/* X0: fields[R3D_RU..R3D_TT] (..R3D_BZ with LMAG), each (npx+2)(npy+2)(npz+2)
+ * doubles, x fastest. */
 int r3d_adj_set_initial(r3d_adj_ctx* a, const double* const* fields);
 /* Run nsteps steps from X0, keeping the checkpoints; J = objective. */
 int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J);
-/* Gradient of the last forward's J: grad[R3D_RU..R3D_TT] (global arrays as
- * X0; ghost entries zero), dJ/dRe, dJ/dtheta.  NULL outputs are skipped. */
+/* Gradient of the last forward's J: grad[R3D_RU..R3D_TT] (..R3D_BZ with LMAG;
+ * global arrays as X0; ghost entries zero), dJ/dRe, dJ/dtheta.  NULL outputs
+ * are skipped. */
 int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double* dJ_dtheta);
 /* Wall milliseconds of the last forward and backward calls. */
 int r3d_adj_timing(r3d_adj_ctx* a, double* forward_ms, double* backward_ms);

@@ -362,6 +362,7 @@ class Adjoint:
         _check(L.r3d_adj_create(ctypes.byref(cfg), checkpoint_every, ctypes.byref(h)))
         self.h = h
         self.shape = (cfg.npz + 2, cfg.npy + 2, cfg.npx + 2)
+        self.names = PRIMARIES[:8 if cfg.lmag else 5]
         try:
             keep = {k: np.ascontiguousarray(metrics[k], dtype=np.float64) for k in METRIC_NAMES}
             st = Metrics(*[_dptr(keep[k]) for k in METRIC_NAMES])
@@ -382,10 +383,10 @@ class Adjoint:
             pass
 
     def set_initial(self, fields):
-        arrs = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in PRIMARIES[:5]]
+        arrs = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in self.names]
         for a in arrs:
             assert a.shape == self.shape
-        ptrs = (ctypes.POINTER(ctypes.c_double) * 5)(*[_dptr(a) for a in arrs])
+        ptrs = (ctypes.POINTER(ctypes.c_double) * len(arrs))(*[_dptr(a) for a in arrs])
         _check(load().r3d_adj_set_initial(self.h, ptrs))
 
     def forward(self, nsteps):
@@ -394,8 +395,9 @@ class Adjoint:
         return J.value
 
     def backward(self):
-        out = {n: np.zeros(self.shape) for n in PRIMARIES[:5]}
-        ptrs = (ctypes.POINTER(ctypes.c_double) * 5)(*[_dptr(out[n]) for n in PRIMARIES[:5]])
+        out = {n: np.zeros(self.shape) for n in self.names}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * len(self.names))(
+            *[_dptr(out[n]) for n in self.names])
         dre, dth = ctypes.c_double(), ctypes.c_double()
         _check(load().r3d_adj_backward(self.h, ptrs, ctypes.byref(dre), ctypes.byref(dth)))
         return out, dre.value, dth.value
@@ -406,8 +408,9 @@ class Adjoint:
         return f.value, b.value
 
     def final_state(self):
-        out = {n: np.zeros(self.shape) for n in PRIMARIES[:5]}
-        ptrs = (ctypes.POINTER(ctypes.c_double) * 5)(*[_dptr(out[n]) for n in PRIMARIES[:5]])
+        out = {n: np.zeros(self.shape) for n in self.names}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * len(self.names))(
+            *[_dptr(out[n]) for n in self.names])
         _check(load().r3d_adj_download_final(self.h, ptrs))
         return out
 

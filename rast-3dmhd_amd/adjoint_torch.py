@@ -4,7 +4,7 @@ KineticEnergyRun(args, nsteps) is the differentiable map
     (tt0, Re, theta) -> J = final kinetic energy of a hydrodynamic run,
 where tt0 is the interior initial temperature (npz, npy, npx) replacing that
 of the run's initial state (built by the host setup from the reference CLI
-flags `args`).  The forward is the plain bit-exact time step on the GPU; the
+flags `args`; hydrodynamic or MHD).  The forward is the plain bit-exact time step on the GPU; the
 backward is the hand-written adjoint.  Composing tt0 = T_static * (1 + delta)
 in torch gives the gradient with respect to the perturbation field delta.
 """
@@ -13,13 +13,11 @@ import torch
 
 from . import Adjoint, PRIMARIES, Run, cfg_with
 
-_PRIM = PRIMARIES[:5]
-
 
 class KineticEnergyRun:
     def __init__(self, args, nsteps, checkpoint_every=20):
         self.run = Run(args)
-        self.base = {k: v for k, v in self.run.state().items() if k in _PRIM}
+        self.base = dict(self.run.state())
         self.metrics = self.run.metrics()
         self.cfg = self.run.cfg()
         self.nsteps = nsteps

@@ -36,7 +36,7 @@ namespace {
 
 constexpr int AT = 256;          // threads per block of the adjoint kernels
 constexpr int ADJ_BLOCKS = 2048; // grid-stride blocks of the cell kernels
-constexpr int NF = 5;            // hydrodynamic primaries
+constexpr int NFH = 5;           // hydrodynamic primaries
 
 __device__ __forceinline__ void cell_of(const Geo& g, long q, int& i, int& j, int& k) {
     i = 1 + (int)(q % g.npx);
@@ -74,6 +74,27 @@ __device__ void gather_stencil(const Geo& g, const State& X, const Metrics& M, i
     m.dk = M.dkapa[k];
 }
 
+// B at the stencil points and the LMAG wall extras.
+__device__ void gather_mag(const Geo& g, const State& X, int i, int j, int k, const CellMet& m,
+                           MagStencil& b) {
+    for (int p = 0; p < NPTS; ++p) {
+        int di, dj, dk;
+        pt_offset(p, di, dj, dk);
+        const long n = g.at(wrapx(g, i + di), wrapy(g, j + dj), k + dk);
+        b.p[p] = {X.q[QBX][n], X.q[QBY][n], X.q[QBZ][n]};
+    }
+    b.ro2 = b.bx2 = b.by2 = b.bx3 = b.by3 = 0.0;
+    if (m.wall == 1 || m.mag_bot) {
+        const int dir = m.wall == 1 ? 1 : -1;
+        const long n2 = g.at(i, j, k + 2 * dir), n3 = g.at(i, j, k + 3 * dir);
+        b.ro2 = X.q[QRO][n2];
+        b.bx2 = X.q[QBX][n2];
+        b.by2 = X.q[QBY][n2];
+        b.bx3 = X.q[QBX][n3];
+        b.by3 = X.q[QBY][n3];
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -97,10 +118,12 @@ __device__ void block_sum(double v, double* red, double* acc) {
 // LO += LB (the stage input's own adjoint, passed through) + (dF/dX)^T lamF.
 // acc[0] += d(lamF.F)/d(ore), acc[1] += d(lamF.F)/d(tk), acc[2] += <F, lamF>.
 // LO must not alias LA or LB.
+template <bool MAG>
 __global__ __launch_bounds__(AT) void k_flux_vjp(const Geo g, const State X, const Consts C,
                                                  const Metrics M, const State LA, double ca,
                                                  const State LB, double cb, const State LO,
                                                  double* acc) {
+    constexpr int NF = MAG ? 8 : 5;
     __shared__ double red[AT / 64];
     double go = 0.0, gt = 0.0, dot = 0.0;
     const long total = (long)g.npx * g.npy * g.npz;
@@ -108,21 +131,32 @@ __global__ __launch_bounds__(AT) void k_flux_vjp(const Geo g, const State X, con
         int i, j, k;
         cell_of(g, q, i, j, k);
         const long n = g.at(i, j, k);
-        double lam[5], own[5];
-        for (int f = 0; f < 5; ++f) {
+        double lam[NF], own[NF];
+        for (int f = 0; f < NF; ++f) {
             own[f] = LB.q[f][n];
             lam[f] = cb * own[f] + (ca != 0.0 ? ca * LA.q[f][n] : 0.0);
         }
         Stencil s, A;
         CellMet m;
         gather_stencil(g, X, M, i, j, k, s, m);
-        double F[5];
-        cell_flux(s, m, C, F);
-        for (int f = 0; f < 5; ++f) dot += F[f] * lam[f];
-        double a, b;
-        cell_flux_vjp(s, m, C, lam, A, a, b);
+        m.mag_bot = m.wall == 2 && C.mag_bot;
+        MagStencil b, AB;
+        double F[NF];
+        double a, bb;
+        if constexpr (MAG) {
+            gather_mag(g, X, i, j, k, m, b);
+            cell_flux_mag(s, b, m, C, F);
+            cell_flux_mag_vjp(s, b, m, C, lam, A, AB, a, bb);
+            AB.p[P_C].bx += own[QBX];
+            AB.p[P_C].by += own[QBY];
+            AB.p[P_C].bz += own[QBZ];
+        } else {
+            cell_flux(s, m, C, F);
+            cell_flux_vjp(s, m, C, lam, A, a, bb);
+        }
+        for (int f = 0; f < NF; ++f) dot += F[f] * lam[f];
         go += a;
-        gt += b;
+        gt += bb;
         A.p[P_C].ru += own[QRU];
         A.p[P_C].rv += own[QRV];
         A.p[P_C].rw += own[QRW];
@@ -139,8 +173,22 @@ __global__ __launch_bounds__(AT) void k_flux_vjp(const Geo g, const State X, con
             atomicAdd(&LO.q[QRW][t], A.p[p].rw);
             atomicAdd(&LO.q[QRO][t], A.p[p].ro);
             atomicAdd(&LO.q[QTT][t], A.p[p].tt);
+            if constexpr (MAG) {
+                atomicAdd(&LO.q[QBX][t], AB.p[p].bx);
+                atomicAdd(&LO.q[QBY][t], AB.p[p].by);
+                atomicAdd(&LO.q[QBZ][t], AB.p[p].bz);
+            }
         }
         if (m.wall) atomicAdd(&LO.q[QRW][g.at(i, j, m.wall == 1 ? k + 2 : k - 2)], A.rw2);
+        if (MAG && (m.wall == 1 || m.mag_bot)) {
+            const int dir = m.wall == 1 ? 1 : -1;
+            const long n2 = g.at(i, j, k + 2 * dir), n3 = g.at(i, j, k + 3 * dir);
+            atomicAdd(&LO.q[QRO][n2], AB.ro2);
+            atomicAdd(&LO.q[QBX][n2], AB.bx2);
+            atomicAdd(&LO.q[QBY][n2], AB.by2);
+            atomicAdd(&LO.q[QBX][n3], AB.bx3);
+            atomicAdd(&LO.q[QBY][n3], AB.by3);
+        }
     }
     block_sum(go, red, &acc[0]);
     block_sum(gt, red, &acc[1]);
@@ -174,6 +222,7 @@ __global__ __launch_bounds__(AT) void k_bcon_adj(const Geo g, const State X, con
             LX.q[f][n] = 0.0;
         }
         LX.q[QRW][n] = 0.0;
+        if (C.lmag && C.bz_zero) LX.q[QBZ][n] = 0.0;  // BZ = 0 on the walls
         const double aT = LX.q[QTT][n];
         bool fixed = false, extrap = false;
         double dth = 0.0;
@@ -261,10 +310,17 @@ __global__ __launch_bounds__(AT) void k_dt_argmin(const Geo g, const State X, co
         const double ro = X.q[QRO][n], tt = X.q[QTT][n];
         const double dd = dmin(dmin(M.ddx[i], M.ddy[j]), M.ddz[k]);
         const double uu = (ru * ru + rv_ * rv_ + rw * rw) * (1.0 / ro) * (1.0 / ro);
-        const double fast = uu + C.gamma * tt + 2.0 * sqrt(uu * C.gamma * tt);
-        const double L[3] = {dd / sqrt(fast), 0.5 * dd * dd * C.repr * C.cv * ro / M.rkapa[k],
-                             0.375 * dd * dd * C.re * ro};
-        for (int t = 0; t < 3; ++t) {
+        double fast;
+        if (C.lmag) {
+            const double bx = X.q[QBX][n], by = X.q[QBY][n], bz = X.q[QBZ][n];
+            const double b2 = (bx * bx + by * by + bz * bz) * C.obeta / ro;
+            fast = uu + C.gamma * tt + b2 + 2.0 * sqrt(uu * (C.gamma * tt + b2));
+        } else {
+            fast = uu + C.gamma * tt + 2.0 * sqrt(uu * C.gamma * tt);
+        }
+        const double L[4] = {dd / sqrt(fast), 0.5 * dd * dd * C.repr * C.cv * ro / M.rkapa[k],
+                             0.375 * dd * dd * C.re * ro, 0.5 * dd * dd * C.rm};
+        for (int t = 0; t < (C.lmag ? 4 : 3); ++t) {
             const long kk = q * 4 + t + 1;
             if (L[t] < best || (L[t] == best && kk < key)) {
                 best = L[t];
@@ -328,27 +384,39 @@ __global__ __launch_bounds__(64) void k_dt_apply(const Geo g, const State X, con
     const double ru = X.q[QRU][n], rv = X.q[QRV][n], rw = X.q[QRW][n];
     const double ro = X.q[QRO][n], tt = X.q[QTT][n];
     const double L = best, s = C.sff * acc[2] / dt;
-    double dv[NF] = {0.0, 0.0, 0.0, 0.0, 0.0}, dre = 0.0;
+    double dv[NXF] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, dre = 0.0;
     if (t == 0) {
-        // L = dd / sqrt(fast), fast = uu + gT + 2 sqrt(uu gT), uu = |m|^2 / ro^2
+        // L = dd / sqrt(fast), fast = uu + a + 2 sqrt(uu a), uu = |m|^2 / ro^2,
+        // a = gamma T (+ b2 = |B|^2 obeta / ro with LMAG)
         const double m2 = ru * ru + rv * rv + rw * rw;
-        const double uu = m2 / (ro * ro), gT = C.gamma * tt;
-        const double sq = sqrt(uu * gT);
-        const double fast = uu + gT + 2.0 * sq;
+        const double uu = m2 / (ro * ro);
+        double bx = 0.0, by = 0.0, bz = 0.0, b2 = 0.0;
+        if (C.lmag) {
+            bx = X.q[QBX][n];
+            by = X.q[QBY][n];
+            bz = X.q[QBZ][n];
+            b2 = (bx * bx + by * by + bz * bz) * C.obeta / ro;
+        }
+        const double aa = C.gamma * tt + b2;
+        const double sq = sqrt(uu * aa);
+        const double fast = uu + aa + 2.0 * sq;
         const double dLdf = -0.5 * L / fast;
-        const double dfdu = uu > 0.0 ? 1.0 + gT / sq : 0.0;
-        const double dfdT = C.gamma * (1.0 + (uu > 0.0 ? uu / sq : 0.0));
+        const double dfdu = uu > 0.0 ? 1.0 + aa / sq : 0.0;
+        const double dfda = 1.0 + (uu > 0.0 ? uu / sq : 0.0);
         dv[QRU] = dLdf * dfdu * 2.0 * ru / (ro * ro);
         dv[QRV] = dLdf * dfdu * 2.0 * rv / (ro * ro);
         dv[QRW] = dLdf * dfdu * 2.0 * rw / (ro * ro);
-        dv[QRO] = dLdf * dfdu * (-2.0 * uu / ro);
-        dv[QTT] = dLdf * dfdT;
-    } else {
+        dv[QRO] = dLdf * (dfdu * (-2.0 * uu / ro) + dfda * (-b2 / ro));
+        dv[QTT] = dLdf * dfda * C.gamma;
+        dv[QBX] = dLdf * dfda * 2.0 * bx * C.obeta / ro;
+        dv[QBY] = dLdf * dfda * 2.0 * by * C.obeta / ro;
+        dv[QBZ] = dLdf * dfda * 2.0 * bz * C.obeta / ro;
+    } else if (t < 3) {
         // thermal 0.5 dd^2 Re Pr cv ro / rk and viscous 0.375 dd^2 Re ro: L ~ Re ro
         dv[QRO] = L / ro;
         dre = L / C.re;
-    }
-    for (int f = 0; f < NF; ++f) LX.q[f][n] += s * dv[f];
+    }  // t = 3: the resistive limit 0.5 dd^2 Rm depends on neither
+    for (int f = 0; f < (C.lmag ? 8 : 5); ++f) LX.q[f][n] += s * dv[f];
     acc[4] += s * dre;
 }
 
@@ -358,6 +426,7 @@ struct r3d_adj_ctx {
     r3d_hip_ctx* work = nullptr;   // forward steps (plain path)
     int K = 20;                     // checkpoint cadence
     int nsteps = 0;
+    int nf = 5;                     // primaries: 5, or 8 with LMAG
     size_t fe = 0;                  // doubles per field array
     std::vector<double*> ckpt;      // X at steps 0, K, 2K, ...
     std::vector<double*> segx, seg1, seg2;  // X_n, X1, X2 of the steps of one segment
@@ -377,24 +446,24 @@ namespace {
 
 State state_of(const r3d_adj_ctx* a, double* base) {
     State s{};
-    for (int f = 0; f < NF; ++f) s.q[f] = base + f * a->fe;
+    for (int f = 0; f < a->nf; ++f) s.q[f] = base + f * a->fe;
     return s;
 }
 
 int alloc_state(r3d_adj_ctx* a, double** p) {
-    if (hipMalloc((void**)p, a->fe * NF * sizeof(double)) != hipSuccess) {
+    if (hipMalloc((void**)p, a->fe * a->nf * sizeof(double)) != hipSuccess) {
         *p = nullptr;
         return set_err(R3D_ENOMEM, "adjoint: state buffer of %zu bytes",
-                       a->fe * NF * sizeof(double));
+                       a->fe * a->nf * sizeof(double));
     }
-    a->bytes += a->fe * NF * sizeof(double);
+    a->bytes += a->fe * a->nf * sizeof(double);
     return R3D_OK;
 }
 
 // work context's current X <-> a state buffer
 int put_state(r3d_adj_ctx* a, const double* src) {
     r3d_hip_ctx* w = a->work;
-    for (int f = 0; f < NF; ++f)
+    for (int f = 0; f < a->nf; ++f)
         HIPCHK(hipMemcpyAsync(w->xs[w->cur][f], src + f * a->fe, a->fe * sizeof(double),
                               hipMemcpyDeviceToDevice, w->stream));
     w->next_valid = false;
@@ -402,7 +471,7 @@ int put_state(r3d_adj_ctx* a, const double* src) {
 }
 int get_state(r3d_adj_ctx* a, double* dst) {
     r3d_hip_ctx* w = a->work;
-    for (int f = 0; f < NF; ++f)
+    for (int f = 0; f < a->nf; ++f)
         HIPCHK(hipMemcpyAsync(dst + f * a->fe, w->xs[w->cur][f], a->fe * sizeof(double),
                               hipMemcpyDeviceToDevice, w->stream));
     return R3D_OK;
@@ -433,10 +502,16 @@ int objective(r3d_adj_ctx* a, const double* x, double* lx, double* J) {
 int vjp(r3d_adj_ctx* a, const double* x, const double* la, double ca, const double* lb, double cb,
         double* lo) {
     r3d_hip_ctx* w = a->work;
-    HIPCHK(hipMemsetAsync(lo, 0, a->fe * NF * sizeof(double), w->stream));
-    hipLaunchKernelGGL(k_flux_vjp, dim3(blocks_for(w->g)), dim3(AT), 0, w->stream, w->g,
-                       state_of(a, (double*)x), w->C, w->M, state_of(a, (double*)(la ? la : lb)),
-                       la ? ca : 0.0, state_of(a, (double*)lb), cb, state_of(a, lo), a->acc);
+    HIPCHK(hipMemsetAsync(lo, 0, a->fe * a->nf * sizeof(double), w->stream));
+    const dim3 grid(blocks_for(w->g)), blk(AT);
+    const State sx = state_of(a, (double*)x), sa = state_of(a, (double*)(la ? la : lb));
+    const State sb = state_of(a, (double*)lb), so = state_of(a, lo);
+    if (a->nf == 8)
+        hipLaunchKernelGGL(k_flux_vjp<true>, grid, blk, 0, w->stream, w->g, sx, w->C, w->M, sa,
+                           la ? ca : 0.0, sb, cb, so, a->acc);
+    else
+        hipLaunchKernelGGL(k_flux_vjp<false>, grid, blk, 0, w->stream, w->g, sx, w->C, w->M, sa,
+                           la ? ca : 0.0, sb, cb, so, a->acc);
     HIPCHK(hipGetLastError());
     return R3D_OK;
 }
@@ -469,10 +544,10 @@ extern "C" {
 int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** out) {
     if (!cfg || !out || checkpoint_every < 1) return set_err(R3D_EINVAL, "bad arguments");
     *out = nullptr;
-    if (cfg->lmag) return set_err(R3D_EUNSUP, "adjoint: LMAG runs are not supported");
     if (cfg->tc != 0.0) return set_err(R3D_EUNSUP, "adjoint: tc != 0 is not supported");
     auto* a = new r3d_adj_ctx();
     a->K = checkpoint_every;
+    a->nf = cfg->lmag ? 8 : 5;
     int rc = r3d_hip_create(cfg, &a->work);
     if (rc) {
         delete a;
@@ -500,7 +575,7 @@ int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** o
     a->segdt.assign(a->K, 0.0);
     // ghost entries of the initial state and the adjoint fields stay zero
     for (double* p : {a->x0, a->xfinal, a->lam[0], a->lam[1], a->lam[2]})
-        if (hipMemset(p, 0, a->fe * NF * sizeof(double)) != hipSuccess)
+        if (hipMemset(p, 0, a->fe * a->nf * sizeof(double)) != hipSuccess)
             return fail(set_err(R3D_EHIP, "adjoint: memset"));
     *out = a;
     return R3D_OK;
@@ -530,7 +605,7 @@ int r3d_adj_set_initial(r3d_adj_ctx* a, const double* const* fields) {
     if (!a || !fields) return set_err(R3D_EINVAL, "null argument");
     r3d_hip_ctx* w = a->work;
     const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
-    for (int f = 0; f < NF; ++f) {
+    for (int f = 0; f < a->nf; ++f) {
         if (!fields[f]) return set_err(R3D_EINVAL, "field %d missing", f);
         HIPCHK(hipMemcpy2DAsync(a->x0 + f * a->fe + XOFF, w->g.px * sizeof(double), fields[f],
                                 W * sizeof(double), W * sizeof(double), rows,
@@ -581,7 +656,8 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
     double* Mb = a->lam[1];
     double* Nb = a->lam[2];
     double Jdummy;
-    if ((rc = objective(a, a->xfinal, L, &Jdummy))) return rc;  // L = dJ/dX_N
+    HIPCHK(hipMemsetAsync(L, 0, a->fe * a->nf * sizeof(double), w->stream));
+    if ((rc = objective(a, a->xfinal, L, &Jdummy))) return rc;  // L = dJ/dX_N (B: 0)
     HIPCHK(hipMemsetAsync(a->acc, 0, 5 * sizeof(double), w->stream));
     const double g1 = C.gam[0], g2 = C.gam[1], g3 = C.gam[2];
     const double z1 = C.zeta[0], z2 = C.zeta[1];
@@ -625,7 +701,7 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
     const double g_re = acc[0] * (-C.ore * C.ore) + acc[1] * (-C.tk / C.re) + acc[4];
     if (grad) {
         const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
-        for (int f = 0; f < NF; ++f)
+        for (int f = 0; f < a->nf; ++f)
             if (grad[f])
                 HIPCHK(hipMemcpy2D(grad[f], W * sizeof(double), L + f * a->fe + XOFF,
                                    w->g.px * sizeof(double), W * sizeof(double), rows,
@@ -649,7 +725,7 @@ int r3d_adj_download_final(r3d_adj_ctx* a, double* const* fields) {
     if (!a || !fields) return set_err(R3D_EINVAL, "null argument");
     r3d_hip_ctx* w = a->work;
     const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
-    for (int f = 0; f < NF; ++f)
+    for (int f = 0; f < a->nf; ++f)
         if (fields[f])
             HIPCHK(hipMemcpy2D(fields[f], W * sizeof(double), a->xfinal + f * a->fe + XOFF,
                                w->g.px * sizeof(double), W * sizeof(double), rows,

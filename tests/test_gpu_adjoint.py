@@ -133,3 +133,55 @@ def test_torch_function_gradient_of_the_perturbation():
     say(f"[gpu]   autograd {ad:.12e} fd {fd:.12e}; dJ/dRe {float(re.grad):.6e}")
     assert abs(ad - fd) <= 1e-6 * abs(fd)
     assert re.grad is not None and th.grad is not None
+
+
+MAG_ARGS = ["--npx", "16", "--npy", "16", "--npz", "32", "--npey", "2", "--npez", "2",
+            "--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2",
+            "--bzp", "10", "--lrot", "1", "--ry", "0.1", "--ang", "0"]
+MPRIM = PRIM + ["bx", "by", "bz"]
+
+
+@pytest.fixture(scope="module")
+def msetup():
+    """A developed MHD state with a three-dimensional field as X0."""
+    run = pkg.Run(MAG_ARGS)
+    sim = run.make_sim()
+    sim.step(6)
+    x0 = sim.download_global(MPRIM)
+    sim.close()
+    rng = np.random.default_rng(4)
+    for n in ("bx", "by", "bz"):
+        x0[n][1:-1, 1:-1, 1:-1] += 0.05 * rng.uniform(-1, 1, x0[n][1:-1, 1:-1, 1:-1].shape)
+    return run, x0
+
+
+@pytest.mark.parametrize("field", MPRIM + ["re", "theta"])
+def test_mhd_gradient_matches_finite_differences(msetup, field):
+    run, x0 = msetup
+    cfg, met = run.cfg(), run.metrics()
+    say(f"[gpu] MHD adjoint dJ/d{field}")
+    adj = pkg.Adjoint(cfg, met, EVERY)
+    adj.set_initial(x0)
+    adj.forward(NSTEPS)
+    grads, dre, dth = adj.backward()
+    adj.close()
+    if field in ("re", "theta"):
+        v0 = cfg.re if field == "re" else cfg.theta
+        h = 1e-6 * v0
+        fd = (run_J(pkg.cfg_with(cfg, **{field: v0 + h}), met, x0) -
+              run_J(pkg.cfg_with(cfg, **{field: v0 - h}), met, x0)) / (2 * h)
+        ad = dre if field == "re" else dth
+    else:
+        rng = np.random.default_rng(len(field))
+        d = np.zeros_like(x0[field])
+        d[1:-1, 1:-1, 1:-1] = rng.uniform(-1, 1, d[1:-1, 1:-1, 1:-1].shape)
+        d *= np.abs(x0[field][1:-1, 1:-1, 1:-1]).max()
+        eps = 1e-6
+        xp = {k: v.copy() for k, v in x0.items()}
+        xm = {k: v.copy() for k, v in x0.items()}
+        xp[field] += eps * d
+        xm[field] -= eps * d
+        fd = (run_J(cfg, met, xp) - run_J(cfg, met, xm)) / (2 * eps)
+        ad = float(np.sum(grads[field] * d))
+    say(f"[gpu]   adjoint {ad:.12e} fd {fd:.12e}")
+    assert abs(ad - fd) <= 1e-6 * abs(fd), (field, ad, fd)
