@@ -13,8 +13,8 @@
  * that attain the minimum.
  *
  * Memory: the forward keeps a checkpoint every `checkpoint_every` steps; the
- * backward pass recomputes the forward states of one segment at a time from
- * its checkpoint, then the two intermediate stage states of each step.
+ * backward pass recomputes the states of one segment at a time from its
+ * checkpoint, keeping the two intermediate stage states of every step.
  *
  * Hydrodynamic and MHD runs (with LMAG the state and the gradient have the
  * eight primaries R3D_RU..R3D_BZ); tc = 0.  Errors follow r3d_hip.h.
