@@ -123,14 +123,20 @@ def test_ensemble_workload_64_members_200_steps():
     """The 64-member workload (sizes 48..96 x 96..192, Re, theta, seeds drawn
     by ensemble_members), 200 steps: every member bit-identical to its own
     plain run."""
+    import time
     members = pkg.ensemble_members(64)
     say("[gpu] 64-member ensemble, 200 steps")
+    t0 = time.perf_counter()
     ens = pkg.Ensemble([pkg.Run(a) for a in members])
-    ens.autotune(2)
-    ens.step(200)
-    ens.sync()
+    say(f"[gpu]   set up in {time.perf_counter() - t0:.1f} s")
+    ms = ens.autotune(2)
+    say(f"[gpu]   autotuned {ens.tiles()} {ms:.3f} ms/step, {time.perf_counter() - t0:.1f} s")
+    for done in range(0, 200, 50):
+        ens.step(50)
+        ens.sync()
+        say(f"[gpu]   {done + 50} steps, {time.perf_counter() - t0:.1f} s")
     for m, a in enumerate(members):
-        if m % 16 == 0:
-            say(f"[gpu]   member {m}")
         compare_member(ens, m, *single_run(a, 200), "64-member workload")
+        if m % 8 == 7:
+            say(f"[gpu]   members 0..{m} identical, {time.perf_counter() - t0:.1f} s")
     ens.close()
