@@ -99,7 +99,7 @@ def run_adjoint(a):
     """Adjoint:forward time ratio of the hand-written discrete adjoint."""
     pkg = load_package()
     args = ["--npx", str(a.npx), "--npy", str(a.npy), "--npz", str(a.npz),
-            "--npey", str(a.npey), "--npez", str(a.npez)]
+            "--npey", str(a.npey), "--npez", str(a.npez)] + (MHD_FLAGS if a.mhd else [])
     run = pkg.Run(args)
     cells = a.npx * a.npy * a.npz
     adj = pkg.Adjoint(run.cfg(), run.metrics(), 20)
@@ -115,8 +115,8 @@ def run_adjoint(a):
         "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": bwd_ms / a.steps,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: the reference's static polytrope with its seeded ran2 perturbation",
-        "config": {"workload": f"hydro adjoint {a.npx}x{a.npy}x{a.npz}, checkpoints every 20 "
-                               "steps, J = final kinetic energy",
+        "config": {"workload": f"{'MHD' if a.mhd else 'hydro'} adjoint {a.npx}x{a.npy}x{a.npz}, "
+                               "checkpoints every 20 steps, J = final kinetic energy",
                    "global_cells": cells, "forward_ms": fwd_ms, "backward_ms": bwd_ms,
                    "forward_cell_updates_per_s": cells * 3 * a.steps / (fwd_ms / 1e3)},
         "roofline": None,

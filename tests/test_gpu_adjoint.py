@@ -2,6 +2,16 @@
 the forward time step: dJ/dX0 in random directions of every primary, dJ/dRe,
 dJ/dtheta, and the torch.autograd.Function over the temperature perturbation.
 Relative agreement required: 1e-6.
+
+J depends on the state through dt = sff * min(limits), which is smooth only
+while one cell and limit attain the minimum.  With the default Re = 2,
+Pr = 0.05 the thermal limit (proportional to rho) binds on the top plane,
+where rho is uniform to a few digits: a finite-difference step in rho then
+moves the minimum between near-equal cells and the difference quotient no
+longer measures a derivative.  The state checks therefore run at Re = 20,
+Pr = 1, where the advective limit (set by the hottest cell) binds; the
+default physics is covered by the temperature-perturbation check, whose
+directions leave rho, and with it the binding limit, unchanged.
 """
 import importlib
 import sys
@@ -15,6 +25,7 @@ pkg = load_package()
 pytestmark = pytest.mark.gpu
 
 ARGS = ["--npx", "16", "--npy", "16", "--npz", "32", "--npey", "1", "--npez", "2"]
+SMOOTH = ["--re", "20", "--pr", "1"]  # the advective dt limit binds
 NSTEPS, EVERY = 6, 4
 PRIM = ["ru", "rv", "rw", "ro", "tt"]
 
@@ -27,7 +38,7 @@ def say(msg):
 @pytest.fixture(scope="module")
 def setup():
     """A developed state (10 plain steps from the initial condition) as X0."""
-    run = pkg.Run(ARGS)
+    run = pkg.Run(ARGS + SMOOTH)
     sim = run.make_sim()
     sim.step(10)
     x0 = sim.download_global(PRIM)
@@ -137,7 +148,7 @@ def test_torch_function_gradient_of_the_perturbation():
 
 MAG_ARGS = ["--npx", "16", "--npy", "16", "--npz", "32", "--npey", "2", "--npez", "2",
             "--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2",
-            "--bzp", "10", "--lrot", "1", "--ry", "0.1", "--ang", "0"]
+            "--bzp", "10", "--lrot", "1", "--ry", "0.1", "--ang", "0"] + SMOOTH
 MPRIM = PRIM + ["bx", "by", "bz"]
 
 
