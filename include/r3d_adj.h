@@ -42,6 +42,11 @@ int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J);
  * global arrays as X0; ghost entries zero), dJ/dRe, dJ/dtheta.  NULL outputs
  * are skipped. */
 int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double* dJ_dtheta);
+/* dt > 0: every step of later forward runs takes this dt instead of the CFL
+ * step, and J is then a smooth function of X0, Re and theta (the CFL step is
+ * a minimum over cells and limits, differentiable only where one of them
+ * attains it).  dt = 0 restores the CFL step.  Invalidates the last forward. */
+int r3d_adj_set_fixed_dt(r3d_adj_ctx* a, double dt);
 /* Wall milliseconds of the last forward and backward calls. */
 int r3d_adj_timing(r3d_adj_ctx* a, double* forward_ms, double* backward_ms);
 /* Final state of the last forward (as r3d_hip_download_global). */

@@ -112,6 +112,7 @@ def load():
         "r3d_adj_backward": [vp, ctypes.POINTER(dp), ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_double)],
         "r3d_adj_timing": [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+        "r3d_adj_set_fixed_dt": [vp, ctypes.c_double],
         "r3d_adj_download_final": [vp, ctypes.POINTER(dp)],
     }
     for name, args in sigs.items():
@@ -401,6 +402,11 @@ class Adjoint:
         dre, dth = ctypes.c_double(), ctypes.c_double()
         _check(load().r3d_adj_backward(self.h, ptrs, ctypes.byref(dre), ctypes.byref(dth)))
         return out, dre.value, dth.value
+
+    def set_fixed_dt(self, dt):
+        """dt > 0: later forward runs step with this dt instead of the CFL
+        step (J becomes smooth in X0, Re, theta); 0: back to the CFL step."""
+        _check(load().r3d_adj_set_fixed_dt(self.h, float(dt)))
 
     def timing(self):
         f, b = ctypes.c_double(), ctypes.c_double()

@@ -691,7 +691,7 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
             // stage 1: lamF = z1 dt lamZ1 + g1 dt lamX1; Z0 = X_n
             if ((rc = bcon_adj(a, x1b, Nb))) return rc;
             if ((rc = vjp(a, xn, Mb, z1 * dt, Nb, g1 * dt, L))) return rc;
-            if ((rc = dt_adjoint(a, xn, dt, L))) return rc;
+            if (C.dtfix <= 0.0 && (rc = dt_adjoint(a, xn, dt, L))) return rc;
         }
     }
     double acc[5];
@@ -711,6 +711,14 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
     if (dJ_dth) *dJ_dth = acc[3];
     a->bwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
                     .count();
+    return R3D_OK;
+}
+
+int r3d_adj_set_fixed_dt(r3d_adj_ctx* a, double dt) {
+    if (!a) return set_err(R3D_EINVAL, "null adjoint context");
+    if (!(dt >= 0.0) || !std::isfinite(dt)) return set_err(R3D_EINVAL, "fixed dt %g", dt);
+    a->work->C.dtfix = dt;
+    a->nsteps = 0;  // a forward run under the new rule is needed before backward
     return R3D_OK;
 }
 

@@ -204,7 +204,7 @@ __device__ __forceinline__ void finish_block(DevScal* sc, const double* partial,
             sc->umach_next = sqrt(o[0]);
             double m = dmin(dmin(o[1], o[2]), o[3]);
             if (C.lmag) m = dmin(m, o[4]);
-            sc->dt_next = C.sff * m;
+            sc->dt_next = C.dtfix > 0.0 ? C.dtfix : C.sff * m;
         }
     }
 }

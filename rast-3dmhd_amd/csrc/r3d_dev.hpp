@@ -54,6 +54,7 @@ struct Consts {
     double ohm;  // obeta*orm*ocv     (golden/fortran/3dmhdsub.f:2307)
     double theta, tb, hh, tpr, tp;
     double gam[3], zeta[2];
+    double dtfix;  // > 0: every step takes this dt instead of the CFL one (adjoint runs)
     int lrot, lmag, top_mode, half_t, izcon, bz_zero;
     int mag_bot;  // LMAG one-sided z stencils on the bottom wall (npez > 1,
                   // golden/fortran/3dmhdsub.f:1126, 1392, 1513 sit in an ELSE IF)

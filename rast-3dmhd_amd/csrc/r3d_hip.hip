@@ -96,6 +96,7 @@ namespace {
 void fill_consts(r3d_hip_ctx* c) {
     const r3d_hip_cfg& f = c->cfg;
     Consts& C = c->C;
+    C.dtfix = 0.0;
     C.c13 = f.c13;
     C.c23 = f.c23;
     C.c43 = f.c43;

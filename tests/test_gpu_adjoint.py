@@ -1,17 +1,26 @@
 """Discrete adjoint (include/r3d_adj.h) against central finite differences of
-the forward time step: dJ/dX0 in random directions of every primary, dJ/dRe,
+the forward time step: dJ/dX0 along directions of every primary, dJ/dRe,
 dJ/dtheta, and the torch.autograd.Function over the temperature perturbation.
 Relative agreement required: 1e-6.
 
-J depends on the state through dt = sff * min(limits), which is smooth only
-while one cell and limit attain the minimum.  With the default Re = 2,
-Pr = 0.05 the thermal limit (proportional to rho) binds on the top plane,
-where rho is uniform to a few digits: a finite-difference step in rho then
-moves the minimum between near-equal cells and the difference quotient no
-longer measures a derivative.  The state checks therefore run at Re = 20,
-Pr = 1, where the advective limit (set by the hottest cell) binds; the
-default physics is covered by the temperature-perturbation check, whose
-directions leave rho, and with it the binding limit, unchanged.
+Two properties of J set how the differences are taken.
+* Rounding noise: J (about 2e4 here) carries a rounding noise of a few parts in
+  1e15 that changes with every perturbation of X0, so a central difference
+  with step eps has an absolute error of roughly 1e-11 / eps on top of the
+  eps^2 truncation error.  The direction is weighted towards the adjoint
+  gradient, where the directional derivative is largest.  For the velocity,
+  density and temperature the signal is then 1e2 .. 1e6 and eps = 1e-6 (of
+  the field's magnitude) leaves errors near 1e-10; a sweep over eps shows the
+  error falling as eps^2 down to 1e-6.  J is a thousand times less sensitive
+  to the magnetic field and to Re: below eps ~ 3e-4 the noise dominates (error
+  growing as 1 / eps), so those steps are 1e-3 (field) and 1e-4 (Re, theta).
+* The CFL step dt = sff * min(limits) is smooth only while one cell and limit
+  attain the minimum.  With the default Re = 2, Pr = 0.05 the thermal limit
+  (proportional to rho) binds on the top plane, where rho is uniform to a few
+  digits, and a step in rho moves the minimum between near-equal cells.  The
+  CFL checks therefore run at Re = 20, Pr = 1, where the advective limit (set
+  by the hottest cell) binds; the default physics is checked with a fixed dt
+  (r3d_adj_set_fixed_dt), which makes J smooth everywhere.
 """
 import importlib
 import sys
@@ -27,6 +36,9 @@ pytestmark = pytest.mark.gpu
 ARGS = ["--npx", "16", "--npy", "16", "--npz", "32", "--npey", "1", "--npez", "2"]
 SMOOTH = ["--re", "20", "--pr", "1"]  # the advective dt limit binds
 NSTEPS, EVERY = 6, 4
+# finite-difference steps, relative to the field's magnitude or the parameter
+EPS = {"ru": 1e-6, "rv": 1e-6, "rw": 1e-6, "ro": 1e-6, "tt": 1e-6,
+       "bx": 1e-3, "by": 1e-3, "bz": 1e-3, "re": 1e-4, "theta": 1e-4}
 PRIM = ["ru", "rv", "rw", "ro", "tt"]
 
 
@@ -46,12 +58,71 @@ def setup():
     return run, x0
 
 
-def run_J(cfg, metrics, x0, nsteps=NSTEPS):
+@pytest.fixture(scope="module")
+def fixed_setup():
+    """Default physics, a developed state, and 0.8 of its CFL step as the
+    fixed dt."""
+    run = pkg.Run(ARGS)
+    sim = run.make_sim()
+    sim.step(10)
+    x0 = sim.download_global(PRIM)
+    dt = 0.8 * sim.scalars()["dt"]
+    sim.close()
+    return run, x0, dt
+
+
+def fd_check(cfg, met, x0, field, grads, dre, dth, fixed_dt=0.0, tol=1e-6):
+    """Central difference of J along a direction of one primary, or in Re /
+    theta, against the adjoint's directional derivative.  The direction is
+    the adjoint gradient scaled to unit maximum plus uniform noise in [-1, 1],
+    times the field's largest magnitude; the step is EPS[field] of that."""
+    eps = EPS[field]
+    if field in ("re", "theta"):
+        v0 = cfg.re if field == "re" else cfg.theta
+        h = eps * v0
+        fd = (run_J(pkg.cfg_with(cfg, **{field: v0 + h}), met, x0, fixed_dt=fixed_dt) -
+              run_J(pkg.cfg_with(cfg, **{field: v0 - h}), met, x0, fixed_dt=fixed_dt)) / (2 * h)
+        ad = dre if field == "re" else dth
+    else:
+        rng = np.random.default_rng(sum(map(ord, field)))
+        g = grads[field][1:-1, 1:-1, 1:-1]
+        d = np.zeros_like(x0[field])
+        d[1:-1, 1:-1, 1:-1] = g / np.abs(g).max() + rng.uniform(-1, 1, g.shape)
+        d *= np.abs(x0[field][1:-1, 1:-1, 1:-1]).max()
+        xp = {k: v.copy() for k, v in x0.items()}
+        xm = {k: v.copy() for k, v in x0.items()}
+        xp[field] += eps * d
+        xm[field] -= eps * d
+        fd = (run_J(cfg, met, xp, fixed_dt=fixed_dt) -
+              run_J(cfg, met, xm, fixed_dt=fixed_dt)) / (2 * eps)
+        ad = float(np.sum(grads[field] * d))
+    say(f"[gpu]   adjoint {ad:.12e} fd {fd:.12e} rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(ad - fd) <= tol * abs(fd), (field, ad, fd)
+
+
+def run_J(cfg, metrics, x0, nsteps=NSTEPS, fixed_dt=0.0):
     adj = pkg.Adjoint(cfg, metrics, EVERY)
+    adj.set_fixed_dt(fixed_dt)
     adj.set_initial(x0)
     J = adj.forward(nsteps)
     adj.close()
     return J
+
+
+@pytest.mark.parametrize("field", PRIM + ["re", "theta"])
+def test_fixed_dt_gradient_matches_finite_differences(fixed_setup, field):
+    """With a fixed step J is smooth: the transpose of the fluxes and the
+    boundary conditions alone, at the default Re = 2, Pr = 0.05."""
+    run, x0, dt = fixed_setup
+    cfg, met = run.cfg(), run.metrics()
+    say(f"[gpu] fixed-dt adjoint dJ/d{field} (dt {dt:.6e})")
+    adj = pkg.Adjoint(cfg, met, EVERY)
+    adj.set_fixed_dt(dt)
+    adj.set_initial(x0)
+    adj.forward(NSTEPS)
+    grads, dre, dth = adj.backward()
+    adj.close()
+    fd_check(cfg, met, x0, field, grads, dre, dth, fixed_dt=dt)
 
 
 def test_adjoint_forward_is_the_plain_path(setup):
@@ -72,50 +143,19 @@ def test_adjoint_forward_is_the_plain_path(setup):
         assert np.array_equal(a, b), n
 
 
-@pytest.mark.parametrize("field", PRIM)
-def test_state_gradient_matches_finite_differences(setup, field):
+@pytest.mark.parametrize("field", PRIM + ["re", "theta"])
+def test_cfl_gradient_matches_finite_differences(setup, field):
     run, x0 = setup
     cfg, met = run.cfg(), run.metrics()
     say(f"[gpu] adjoint dJ/d{field}")
     adj = pkg.Adjoint(cfg, met, EVERY)
     adj.set_initial(x0)
     adj.forward(NSTEPS)
-    grads, _, _ = adj.backward()
+    grads, dre, dth = adj.backward()
     fwd_ms, bwd_ms = adj.timing()
     adj.close()
-    rng = np.random.default_rng(hash(field) % 1000)
-    d = np.zeros_like(x0[field])
-    d[1:-1, 1:-1, 1:-1] = rng.uniform(-1, 1, d[1:-1, 1:-1, 1:-1].shape)
-    d *= np.abs(x0[field][1:-1, 1:-1, 1:-1]).max()
-    eps = 1e-6
-    xp = {k: v.copy() for k, v in x0.items()}
-    xm = {k: v.copy() for k, v in x0.items()}
-    xp[field] += eps * d
-    xm[field] -= eps * d
-    fd = (run_J(cfg, met, xp) - run_J(cfg, met, xm)) / (2 * eps)
-    ad = float(np.sum(grads[field] * d))
-    say(f"[gpu]   adjoint {ad:.12e} fd {fd:.12e} (forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms)")
-    assert abs(ad - fd) <= 1e-6 * abs(fd), (field, ad, fd)
-
-
-@pytest.mark.parametrize("param", ["re", "theta"])
-def test_parameter_gradients_match_finite_differences(setup, param):
-    run, x0 = setup
-    cfg, met = run.cfg(), run.metrics()
-    say(f"[gpu] adjoint dJ/d{param}")
-    adj = pkg.Adjoint(cfg, met, EVERY)
-    adj.set_initial(x0)
-    adj.forward(NSTEPS)
-    _, dre, dth = adj.backward()
-    adj.close()
-    v0 = cfg.re if param == "re" else cfg.theta
-    h = 1e-6 * v0
-    Jp = run_J(pkg.cfg_with(cfg, **{param: v0 + h}), met, x0)
-    Jm = run_J(pkg.cfg_with(cfg, **{param: v0 - h}), met, x0)
-    fd = (Jp - Jm) / (2 * h)
-    ad = dre if param == "re" else dth
-    say(f"[gpu]   adjoint {ad:.12e} fd {fd:.12e}")
-    assert abs(ad - fd) <= 1e-6 * abs(fd), (param, ad, fd)
+    say(f"[gpu]   forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms")
+    fd_check(cfg, met, x0, field, grads, dre, dth)
 
 
 def test_torch_function_gradient_of_the_perturbation():
@@ -176,23 +216,25 @@ def test_mhd_gradient_matches_finite_differences(msetup, field):
     adj.forward(NSTEPS)
     grads, dre, dth = adj.backward()
     adj.close()
-    if field in ("re", "theta"):
-        v0 = cfg.re if field == "re" else cfg.theta
-        h = 1e-6 * v0
-        fd = (run_J(pkg.cfg_with(cfg, **{field: v0 + h}), met, x0) -
-              run_J(pkg.cfg_with(cfg, **{field: v0 - h}), met, x0)) / (2 * h)
-        ad = dre if field == "re" else dth
-    else:
-        rng = np.random.default_rng(len(field))
-        d = np.zeros_like(x0[field])
-        d[1:-1, 1:-1, 1:-1] = rng.uniform(-1, 1, d[1:-1, 1:-1, 1:-1].shape)
-        d *= np.abs(x0[field][1:-1, 1:-1, 1:-1]).max()
-        eps = 1e-6
-        xp = {k: v.copy() for k, v in x0.items()}
-        xm = {k: v.copy() for k, v in x0.items()}
-        xp[field] += eps * d
-        xm[field] -= eps * d
-        fd = (run_J(cfg, met, xp) - run_J(cfg, met, xm)) / (2 * eps)
-        ad = float(np.sum(grads[field] * d))
-    say(f"[gpu]   adjoint {ad:.12e} fd {fd:.12e}")
-    assert abs(ad - fd) <= 1e-6 * abs(fd), (field, ad, fd)
+    fd_check(cfg, met, x0, field, grads, dre, dth)
+
+
+@pytest.mark.parametrize("field", MPRIM + ["re", "theta"])
+def test_mhd_fixed_dt_gradient_matches_finite_differences(msetup, field):
+    run, x0 = msetup
+    cfg, met = run.cfg(), run.metrics()
+    sim = pkg.HipSim(cfg)
+    sim.set_metrics(met)
+    sim.upload_global(x0)
+    sim.set_scalars()
+    sim.step(1)
+    dt = 0.8 * sim.scalars()["dt"]
+    sim.close()
+    say(f"[gpu] MHD fixed-dt adjoint dJ/d{field} (dt {dt:.6e})")
+    adj = pkg.Adjoint(cfg, met, EVERY)
+    adj.set_fixed_dt(dt)
+    adj.set_initial(x0)
+    adj.forward(NSTEPS)
+    grads, dre, dth = adj.backward()
+    adj.close()
+    fd_check(cfg, met, x0, field, grads, dre, dth, fixed_dt=dt)
