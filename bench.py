@@ -105,7 +105,10 @@ def run_adjoint(a):
     adj = pkg.Adjoint(run.cfg(), run.metrics(), 20)
     adj.set_initial(run.state())
     log(f"[bench] adjoint: {a.steps} steps of {args}")
-    adj.forward(min(a.steps, 2))  # warm-up (kernel loading)
+    # warm-up: kernel loading and the first touch of every checkpoint and
+    # segment buffer
+    adj.forward(a.steps)
+    adj.backward()
     J = adj.forward(a.steps)
     adj.backward()
     fwd_ms, bwd_ms = adj.timing()

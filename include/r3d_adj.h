@@ -34,7 +34,10 @@ int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** o
 int r3d_adj_destroy(r3d_adj_ctx* a);
 int r3d_adj_set_metrics(r3d_adj_ctx* a, const r3d_hip_metrics* m);
 /* X0: fields[R3D_RU..R3D_TT] (..R3D_BZ with LMAG), each (npx+2)(npy+2)(npz+2)
- * doubles, x fastest. */
+ * doubles, x fastest.  As with r3d_hip_upload_global, the outer z planes
+ * (k = 0, npz+1) stay as given for the whole run and are read by the wall
+ * cells' fluxes: they must hold finite values with RO > 0, e.g. those of a
+ * downloaded state or of the initial condition. */
 int r3d_adj_set_initial(r3d_adj_ctx* a, const double* const* fields);
 /* Run nsteps steps from X0, keeping the checkpoints; J = objective. */
 int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J);
@@ -47,7 +50,8 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
  * a minimum over cells and limits, differentiable only where one of them
  * attains it).  dt = 0 restores the CFL step.  Invalidates the last forward. */
 int r3d_adj_set_fixed_dt(r3d_adj_ctx* a, double dt);
-/* Wall milliseconds of the last forward and backward calls. */
+/* Wall milliseconds of the last forward and backward calls (backward: up to
+ * the gradient's copy to the host, which is not counted). */
 int r3d_adj_timing(r3d_adj_ctx* a, double* forward_ms, double* backward_ms);
 /* Final state of the last forward (as r3d_hip_download_global). */
 int r3d_adj_download_final(r3d_adj_ctx* a, double* const* fields);

@@ -65,6 +65,22 @@ struct MagStencil {
     double ro2, bx2, by2, bx3, by3;
 };
 
+// Stencil entries of the per-cell transpose that can be nonzero: bit f of
+// VJP_MASK[p] for primary f at point p.  The in-plane and vertical edge
+// points only carry velocity adjoints (u, v at NE..SW; u, w at EU..WD; v, w
+// at NU..SD), which reach the momenta and the density; every other entry of
+// cell_flux_vjp / cell_flux_mag_vjp there is exactly zero.  mag: the whole
+// MHD transpose; lmag_only: cell_flux_mag_extra_vjp alone, which reaches the
+// face points only and never the temperature.
+__host__ __device__ constexpr unsigned vjp_mask(int p, bool mag, bool lmag_only = false) {
+    if (lmag_only) return p <= P_D ? 0xffu & ~(1u << QTT) : 0u;
+    const unsigned all = mag ? 0xffu : 0x1fu;
+    const unsigned UV = (1u << QRU) | (1u << QRV) | (1u << QRO);
+    const unsigned UW = (1u << QRU) | (1u << QRW) | (1u << QRO);
+    const unsigned VW = (1u << QRV) | (1u << QRW) | (1u << QRO);
+    return p <= P_D ? all : (p <= P_SW ? UV : (p <= P_WD ? UW : VW));
+}
+
 // F = (FU, FV, FW, FR, FT) in the order of the primaries QRU..QTT.
 __host__ __device__ inline void cell_flux(const Stencil& s, const CellMet& m, const Consts& C,
                                           double F[5]) {
@@ -513,14 +529,81 @@ __host__ __device__ inline void cell_flux_mag(const Stencil& s, const MagStencil
 // A.rw2 included) and AB (B at the points and the wall extras) receive the
 // derivatives; g_ore, g_tk as for cell_flux_vjp (the LMAG terms do not
 // depend on Re).
-__host__ __device__ inline void cell_flux_mag_vjp(const Stencil& s, const MagStencil& b,
-                                                  const CellMet& m, const Consts& C,
-                                                  const double lam[8], Stencil& A,
-                                                  MagStencil& AB, double& g_ore, double& g_tk) {
-    cell_flux_vjp(s, m, C, lam, A, g_ore, g_tk);
-    double ri[NPTS], u[NPTS], v[NPTS], w[NPTS], bx[NPTS], by[NPTS], bz[NPTS];
-    double au[NPTS], av[NPTS], aw[NPTS];
-    for (int q = 0; q < NPTS; ++q) {
+// The LMAG terms alone (what cell_flux_mag adds to the hydrodynamic F, and
+// the induction fluxes): E = (dFU, dFV, dFW, 0, dFT, WW1, WW2, WW3).  They use
+// the cell and its six face neighbours only (points P_C..P_D) and the wall
+// extras.  Used for <F, lamF>, so the sums are not ordered like the reference.
+constexpr int NFACE = P_D + 1;
+__host__ __device__ inline void cell_flux_mag_extra(const Stencil& s, const MagStencil& b,
+                                                    const CellMet& m, const Consts& C,
+                                                    double E[8]) {
+    double u[NFACE], v[NFACE], w[NFACE], bx[NFACE], by[NFACE], bz[NFACE];
+    for (int q = 0; q < NFACE; ++q) {
+        const double ri = 1.0 / s.p[q].ro;
+        u[q] = s.p[q].ru * ri;
+        v[q] = s.p[q].rv * ri;
+        w[q] = s.p[q].rw * ri;
+        bx[q] = b.p[q].bx;
+        by[q] = b.p[q].by;
+        bz[q] = b.p[q].bz;
+    }
+    const double R = 1.0 / s.p[P_C].ro;
+    const double hxd = C.hx * m.dxx, hyd = C.hy * m.dyy, hzd = C.hz * m.dzz;
+    const double lx1 = C.hx * m.d2xx, lx2 = C.h2x * m.dxx * m.dxx;
+    const double ly1 = C.hy * m.d2yy, ly2 = C.h2y * m.dyy * m.dyy;
+    const double lz1 = C.hz * m.d2zz, lz2 = C.h2z * m.dzz * m.dzz;
+    auto dx = [&](const double* q) { return (q[P_E] - q[P_W]) * hxd; };
+    auto dy = [&](const double* q) { return (q[P_N] - q[P_S]) * hyd; };
+    auto dz = [&](const double* q) { return (q[P_U] - q[P_D]) * hzd; };
+    auto lxy = [&](const double* q) {
+        return (q[P_E] - q[P_W]) * lx1 + (q[P_E] - 2.0 * q[P_C] + q[P_W]) * lx2 +
+               (q[P_N] - q[P_S]) * ly1 + (q[P_N] - 2.0 * q[P_C] + q[P_S]) * ly2;
+    };
+    const double bxy = dy(bx), byx = dx(by), bzx = dx(bz), bxz = dz(bx), byz = dz(by);
+    const double bzy = dy(bz);
+    const double ob = C.obeta;
+    E[QRU] = ob * by[P_C] * (bxy - byx) - ob * bz[P_C] * bzx + ob * bz[P_C] * bxz;
+    E[QRV] = ob * bx[P_C] * (byx - bxy) + ob * bz[P_C] * (byz - bzy);
+    E[QRW] = ob * bx[P_C] * bzx - ob * bx[P_C] * bxz - ob * by[P_C] * (byz - bzy);
+    E[QRO] = 0.0;
+    E[QTT] = (bxy * bxy + byx * byx + bzx * bzx - 2.0 * byx * bxy + bxz * bxz - 2.0 * bxz * bzx +
+              byz * byz + bzy * bzy - 2.0 * byz * bzy) * R * C.ohm;
+    const bool top = m.wall == 1, bot = m.wall == 2 && m.mag_bot;
+    const double w2 = (top || bot) ? s.rw2 * (1.0 / b.ro2) : 0.0;
+    double wzm = dz(w);
+    if (top) wzm = (4.0 * w[P_U] - w2) * hzd;
+    if (bot) wzm = (w2 - 4.0 * w[P_D]) * hzd;
+    auto lz = [&](const double* q, double q2, double q3) {
+        if (top)
+            return (-3.0 * q[P_C] + 4.0 * q[P_U] - q2) * lz1 +
+                   (2.0 * q[P_C] - 5.0 * q[P_U] + 4.0 * q2 - q3) * lz2;
+        if (bot)
+            return (3.0 * q[P_C] - 4.0 * q[P_D] + q2) * lz1 +
+                   (2.0 * q[P_C] - 5.0 * q[P_D] + 4.0 * q2 - q3) * lz2;
+        return (q[P_U] - q[P_D]) * lz1 + (q[P_U] - 2.0 * q[P_C] + q[P_D]) * lz2;
+    };
+    const double u0 = u[P_C], v0 = v[P_C], w0 = w[P_C];
+    E[QBX] = by[P_C] * dy(u) - bx[P_C] * dy(v) + bz[P_C] * dz(u) - bx[P_C] * wzm - u0 * dx(bx) -
+             v0 * dy(bx) - w0 * dz(bx) + C.orm * (lxy(bx) + lz(bx, b.bx2, b.bx3));
+    E[QBY] = bx[P_C] * dx(v) - by[P_C] * dx(u) + bz[P_C] * dz(v) - by[P_C] * wzm - u0 * dx(by) -
+             v0 * dy(by) - w0 * dz(by) + C.orm * (lxy(by) + lz(by, b.by2, b.by3));
+    E[QBZ] = bx[P_C] * dx(w) - bz[P_C] * dx(u) + by[P_C] * dy(w) - bz[P_C] * dy(v) - u0 * dx(bz) -
+             v0 * dy(bz) - w0 * dz(bz) +
+             C.orm * (lxy(bz) + (bz[P_U] - bz[P_D]) * lz1 + (bz[P_U] - 2.0 * bz[P_C] + bz[P_D]) * lz2);
+}
+
+// Transpose of the LMAG terms: adds their adjoint into A at points
+// P_C..P_D (and A.rw2) and sets AB (zero first).  With cell_flux_vjp before
+// it this is the transpose of the whole MHD F.
+__host__ __device__ inline void cell_flux_mag_extra_vjp(const Stencil& s, const MagStencil& b,
+                                                        const CellMet& m, const Consts& C,
+                                                        const double lam[8], Stencil& A,
+                                                        MagStencil& AB) {
+    // the LMAG terms reach the face points only
+    double ri[NFACE], u[NFACE], v[NFACE], w[NFACE], bx[NFACE], by[NFACE], bz[NFACE];
+    double au[NFACE], av[NFACE], aw[NFACE];
+    for (int q = 0; q < NPTS; ++q) AB.p[q].bx = AB.p[q].by = AB.p[q].bz = 0.0;
+    for (int q = 0; q < NFACE; ++q) {
         ri[q] = 1.0 / s.p[q].ro;
         u[q] = s.p[q].ru * ri[q];
         v[q] = s.p[q].rv * ri[q];
@@ -529,7 +612,6 @@ __host__ __device__ inline void cell_flux_mag_vjp(const Stencil& s, const MagSte
         by[q] = b.p[q].by;
         bz[q] = b.p[q].bz;
         au[q] = av[q] = aw[q] = 0.0;
-        AB.p[q].bx = AB.p[q].by = AB.p[q].bz = 0.0;
     }
     AB.ro2 = AB.bx2 = AB.by2 = AB.bx3 = AB.by3 = 0.0;
     const double hxd = C.hx * m.dxx, hyd = C.hy * m.dyy, hzd = C.hz * m.dzz;
@@ -642,7 +724,7 @@ __host__ __device__ inline void cell_flux_mag_vjp(const Stencil& s, const MagSte
     const double q3 = byz * byz + bzy * bzy - 2.0 * byz * bzy;
     const double aR = LT * oh * (q1 + q2 + q3);
     // derived quantities back to the primaries
-    for (int q = 0; q < NPTS; ++q) {
+    for (int q = 0; q < NFACE; ++q) {
         A.p[q].ru += ri[q] * au[q];
         A.p[q].rv += ri[q] * av[q];
         A.p[q].rw += ri[q] * aw[q];
@@ -651,6 +733,16 @@ __host__ __device__ inline void cell_flux_mag_vjp(const Stencil& s, const MagSte
     A.p[P_C].ro -= R * R * aR;
     A.rw2 += ri2 * aw2;
     AB.ro2 -= w2 * ri2 * aw2;
+}
+
+// Transpose of the whole MHD F (cell_flux_mag): the hydrodynamic part, then
+// the LMAG terms.
+__host__ __device__ inline void cell_flux_mag_vjp(const Stencil& s, const MagStencil& b,
+                                                  const CellMet& m, const Consts& C,
+                                                  const double lam[8], Stencil& A,
+                                                  MagStencil& AB, double& g_ore, double& g_tk) {
+    cell_flux_vjp(s, m, C, lam, A, g_ore, g_tk);
+    cell_flux_mag_extra_vjp(s, b, m, C, lam, A, AB);
 }
 
 }  // namespace r3d

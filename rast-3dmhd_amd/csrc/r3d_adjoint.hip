@@ -24,6 +24,8 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "r3d_adj.h"
@@ -53,9 +55,10 @@ __device__ __forceinline__ int wrapy(const Geo& g, int j) {
 
 // The flux stencil of cell (i, j, k) from the global array: periodic x/y
 // neighbours are read from the cells they wrap to, as the stage kernel does.
+template <int NP = NPTS>
 __device__ void gather_stencil(const Geo& g, const State& X, const Metrics& M, int i, int j, int k,
                                Stencil& s, CellMet& m) {
-    for (int p = 0; p < NPTS; ++p) {
+    for (int p = 0; p < NP; ++p) {
         int di, dj, dk;
         pt_offset(p, di, dj, dk);
         const long n = g.at(wrapx(g, i + di), wrapy(g, j + dj), k + dk);
@@ -75,9 +78,10 @@ __device__ void gather_stencil(const Geo& g, const State& X, const Metrics& M, i
 }
 
 // B at the stencil points and the LMAG wall extras.
+template <int NP = NPTS>
 __device__ void gather_mag(const Geo& g, const State& X, int i, int j, int k, const CellMet& m,
                            MagStencil& b) {
-    for (int p = 0; p < NPTS; ++p) {
+    for (int p = 0; p < NP; ++p) {
         int di, dj, dk;
         pt_offset(p, di, dj, dk);
         const long n = g.at(wrapx(g, i + di), wrapy(g, j + dj), k + dk);
@@ -193,6 +197,151 @@ __global__ __launch_bounds__(AT) void k_flux_vjp(const Geo g, const State X, con
     block_sum(go, red, &acc[0]);
     block_sum(gt, red, &acc[1]);
     block_sum(dot, red, &acc[2]);
+}
+
+// Tiled transpose: one workgroup owns a 64 x VT column tile (lane = x, wave =
+// y) and a chunk of kc planes, and marches up the chunk.  Each cell's stencil
+// transpose is added into an LDS ring of three planes (k-1, k, k+1) of the
+// tile plus a one-cell x/y halo; once plane k is done, plane k-1 has received
+// every contribution this workgroup makes to it and is added to LO with one
+// global atomic per nonzero entry (the halo cells and the chunk's edge planes
+// also receive contributions from neighbouring workgroups).  Same results as
+// k_flux_vjp up to the order of the sums.  LO must be zero on entry.
+//
+// LMAG = false: the hydrodynamic transpose (cell_flux_vjp, 19 points) of
+// lamF's first five components; true: the transpose of the LMAG terms alone
+// (cell_flux_mag_extra_vjp, face points), so that an MHD stage is the sum of
+// two launches, each small enough to stay in registers.
+constexpr int VT = 4;  // rows of the tile
+constexpr int VRW = 66, VRH = VT + 2;
+
+template <bool LMAG>
+__global__ __launch_bounds__(64 * VT) void k_vjp_tile(const Geo g, const State X, const Consts C,
+                                                      const Metrics M, const State LA, double ca,
+                                                      const State LB, double cb, const State LO,
+                                                      double* acc, int kc) {
+    constexpr int NF = LMAG ? 8 : 5;      // lamF components used and ring fields
+    constexpr int NP = LMAG ? NFACE : NPTS;
+    constexpr int F0 = LMAG ? QBX : QRU;  // LB passed through for fields F0..NF-1
+    constexpr int SLOT = NF * VRH * VRW;  // doubles of one ring plane
+    __shared__ double ring[3 * SLOT];
+    __shared__ double red[VT];
+    const int ntx = (g.npx + 63) / 64, nty = (g.npy + VT - 1) / VT;
+    const int tx = blockIdx.x % ntx, ty = (blockIdx.x / ntx) % nty, tz = blockIdx.x / (ntx * nty);
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int i = tx * 64 + lane + 1, j = ty * VT + row + 1;
+    const bool act = i <= g.npx && j <= g.npy;
+    const int k0 = 1 + tz * kc, k1 = min(g.npz, k0 + kc - 1);
+    for (int e = threadIdx.x; e < 3 * SLOT; e += 64 * VT) ring[e] = 0.0;
+    auto slot = [&](int kk) { return ((kk - k0 + 1) % 3) * SLOT; };
+    // add ring plane kk to LO and clear it
+    auto flush = [&](int kk) {
+        double* r = ring + slot(kk);
+        for (int e = threadIdx.x; e < SLOT; e += 64 * VT) {
+            const double v = r[e];
+            if (v == 0.0) continue;
+            r[e] = 0.0;
+            const int f = e / (VRH * VRW), rc = e % (VRH * VRW);
+            int gi = tx * 64 + rc % VRW, gj = ty * VT + rc / VRW;  // 0 .. npx+1 / npy+1
+            if (gi > g.npx + 1 || gj > g.npy + 1) continue;  // past a partial tile: never set
+            if (gi == 0) gi = g.npx;
+            else if (gi == g.npx + 1) gi = 1;
+            if (gj == 0) gj = g.npy;
+            else if (gj == g.npy + 1) gj = 1;
+            atomicAdd(&LO.q[f][g.at(gi, gj, kk)], v);
+        }
+    };
+    double go = 0.0, gt = 0.0, dot = 0.0;
+    __syncthreads();
+    for (int k = k0; k <= k1; ++k) {
+        if (act) {
+            const long n = g.at(i, j, k);
+            double lam[8];
+            for (int f = 0; f < NF; ++f)
+                lam[f] = cb * LB.q[f][n] + (ca != 0.0 ? ca * LA.q[f][n] : 0.0);
+            Stencil s, A;
+            CellMet m;
+            gather_stencil<NP>(g, X, M, i, j, k, s, m);
+            m.mag_bot = m.wall == 2 && C.mag_bot;
+            MagStencil b, AB;
+            double F[NF];
+            if constexpr (LMAG) {
+                gather_mag<NP>(g, X, i, j, k, m, b);
+                cell_flux_mag_extra(s, b, m, C, F);
+                for (int p = 0; p < NP; ++p) A.p[p] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                A.rw2 = 0.0;
+                cell_flux_mag_extra_vjp(s, b, m, C, lam, A, AB);
+            } else {
+                double a, bb;
+                cell_flux(s, m, C, F);
+                cell_flux_vjp(s, m, C, lam, A, a, bb);
+                go += a;
+                gt += bb;
+            }
+            for (int f = 0; f < NF; ++f) dot += F[f] * lam[f];
+            double own[8];
+            for (int f = F0; f < NF; ++f) own[f] = LB.q[f][n];
+            if constexpr (LMAG) {
+                AB.p[P_C].bx += own[QBX];
+                AB.p[P_C].by += own[QBY];
+                AB.p[P_C].bz += own[QBZ];
+            } else {
+                A.p[P_C].ru += own[QRU];
+                A.p[P_C].rv += own[QRV];
+                A.p[P_C].rw += own[QRW];
+                A.p[P_C].ro += own[QRO];
+                A.p[P_C].tt += own[QTT];
+            }
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                int di, dj, dk;
+                pt_offset(p, di, dj, dk);
+                const int kk = k + dk;
+                if (kk < 1 || kk > g.npz) continue;  // outer planes are not state
+                double* r = ring + slot(kk) + (row + 1 + dj) * VRW + (lane + 1 + di);
+                const unsigned msk = vjp_mask(p, false, LMAG);
+                const double vals[8] = {A.p[p].ru, A.p[p].rv, A.p[p].rw, A.p[p].ro, A.p[p].tt,
+                                        LMAG ? AB.p[p].bx : 0.0, LMAG ? AB.p[p].by : 0.0,
+                                        LMAG ? AB.p[p].bz : 0.0};
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    if (msk & (1u << f)) atomicAdd(r + f * VRH * VRW, vals[f]);
+            }
+            if (m.wall && (!LMAG || m.wall == 1 || m.mag_bot))
+                atomicAdd(&LO.q[QRW][g.at(i, j, m.wall == 1 ? k + 2 : k - 2)], A.rw2);
+            if (LMAG && (m.wall == 1 || m.mag_bot)) {
+                const int dir = m.wall == 1 ? 1 : -1;
+                const long n2 = g.at(i, j, k + 2 * dir), n3 = g.at(i, j, k + 3 * dir);
+                atomicAdd(&LO.q[QRO][n2], AB.ro2);
+                atomicAdd(&LO.q[QBX][n2], AB.bx2);
+                atomicAdd(&LO.q[QBY][n2], AB.by2);
+                atomicAdd(&LO.q[QBX][n3], AB.bx3);
+                atomicAdd(&LO.q[QBY][n3], AB.by3);
+            }
+        }
+        __syncthreads();
+        if (k - 1 >= 1) flush(k - 1);  // complete for this workgroup
+        __syncthreads();
+    }
+    flush(k1);
+    if (k1 + 1 <= g.npz) flush(k1 + 1);
+    // block sums
+    auto bsum = [&](double v, double* out) {
+        v = wave_sum(v);
+        if (lane == 0) red[row] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < VT; ++w) t += red[w];
+            atomicAdd(out, t);
+        }
+        __syncthreads();
+    };
+    if (!LMAG) {
+        bsum(go, &acc[0]);
+        bsum(gt, &acc[1]);
+    }
+    bsum(dot, &acc[2]);
 }
 
 // LX <- B^T LX on the two wall planes, with X the post-boundary state;
@@ -440,6 +589,7 @@ struct r3d_adj_ctx {
     double J = 0.0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
     unsigned long long bytes = 0;
+    bool tiled = true;  // k_vjp_tile; R3D_ADJ_VJP=atomic selects k_flux_vjp
 };
 
 namespace {
@@ -460,12 +610,16 @@ int alloc_state(r3d_adj_ctx* a, double** p) {
     return R3D_OK;
 }
 
-// work context's current X <-> a state buffer
+// work context's current X <-> a state buffer.  put_state fills both X
+// buffers, as an upload does: the stage kernels write interior cells only, so
+// the outer z planes of every stage state (read by the wall cells' fluxes)
+// are those of the state put here.
 int put_state(r3d_adj_ctx* a, const double* src) {
     r3d_hip_ctx* w = a->work;
-    for (int f = 0; f < a->nf; ++f)
-        HIPCHK(hipMemcpyAsync(w->xs[w->cur][f], src + f * a->fe, a->fe * sizeof(double),
-                              hipMemcpyDeviceToDevice, w->stream));
+    for (int s = 0; s < 2; ++s)
+        for (int f = 0; f < a->nf; ++f)
+            HIPCHK(hipMemcpyAsync(w->xs[s][f], src + f * a->fe, a->fe * sizeof(double),
+                                  hipMemcpyDeviceToDevice, w->stream));
     w->next_valid = false;
     return R3D_OK;
 }
@@ -503,9 +657,27 @@ int vjp(r3d_adj_ctx* a, const double* x, const double* la, double ca, const doub
         double* lo) {
     r3d_hip_ctx* w = a->work;
     HIPCHK(hipMemsetAsync(lo, 0, a->fe * a->nf * sizeof(double), w->stream));
-    const dim3 grid(blocks_for(w->g)), blk(AT);
     const State sx = state_of(a, (double*)x), sa = state_of(a, (double*)(la ? la : lb));
     const State sb = state_of(a, (double*)lb), so = state_of(a, lo);
+    const Geo& g = w->g;
+    if (a->tiled) {
+        // chunks of >= 8 planes, about 2048 workgroups in all
+        const long tiles = (long)((g.npx + 63) / 64) * ((g.npy + VT - 1) / VT);
+        long nkz = std::max<long>(1, std::min<long>(g.npz / 8, (2048 + tiles - 1) / tiles));
+        const int kc = (int)((g.npz + nkz - 1) / nkz);
+        nkz = (g.npz + kc - 1) / kc;
+        const dim3 grid((unsigned)(tiles * nkz)), blk(64 * VT);
+        hipLaunchKernelGGL(k_vjp_tile<false>, grid, blk, 0, w->stream, g, sx, w->C, w->M, sa,
+                           la ? ca : 0.0, sb, cb, so, a->acc, kc);
+        HIPCHK(hipGetLastError());
+        if (a->nf == 8) {
+            hipLaunchKernelGGL(k_vjp_tile<true>, grid, blk, 0, w->stream, g, sx, w->C, w->M, sa,
+                               la ? ca : 0.0, sb, cb, so, a->acc, kc);
+            HIPCHK(hipGetLastError());
+        }
+        return R3D_OK;
+    }
+    const dim3 grid(blocks_for(w->g)), blk(AT);
     if (a->nf == 8)
         hipLaunchKernelGGL(k_flux_vjp<true>, grid, blk, 0, w->stream, w->g, sx, w->C, w->M, sa,
                            la ? ca : 0.0, sb, cb, so, a->acc);
@@ -548,6 +720,7 @@ int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** o
     auto* a = new r3d_adj_ctx();
     a->K = checkpoint_every;
     a->nf = cfg->lmag ? 8 : 5;
+    if (const char* e = std::getenv("R3D_ADJ_VJP")) a->tiled = std::strcmp(e, "atomic") != 0;
     int rc = r3d_hip_create(cfg, &a->work);
     if (rc) {
         delete a;
@@ -603,15 +776,11 @@ int r3d_adj_set_metrics(r3d_adj_ctx* a, const r3d_hip_metrics* m) {
 
 int r3d_adj_set_initial(r3d_adj_ctx* a, const double* const* fields) {
     if (!a || !fields) return set_err(R3D_EINVAL, "null argument");
-    r3d_hip_ctx* w = a->work;
-    const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
     for (int f = 0; f < a->nf; ++f) {
         if (!fields[f]) return set_err(R3D_EINVAL, "field %d missing", f);
-        HIPCHK(hipMemcpy2DAsync(a->x0 + f * a->fe + XOFF, w->g.px * sizeof(double), fields[f],
-                                W * sizeof(double), W * sizeof(double), rows,
-                                hipMemcpyHostToDevice, w->stream));
+        int rc = global_copy(a->work, a->x0 + f * a->fe, const_cast<double*>(fields[f]), false);
+        if (rc) return rc;
     }
-    HIPCHK(hipStreamSynchronize(w->stream));
     a->nsteps = 0;
     return R3D_OK;
 }
@@ -697,20 +866,15 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
     double acc[5];
     HIPCHK(hipMemcpyAsync(acc, a->acc, sizeof acc, hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
-    // ore = 1/Re, tk = ocv/(Re Pr); acc[4] holds the dt limits' d/dRe
-    const double g_re = acc[0] * (-C.ore * C.ore) + acc[1] * (-C.tk / C.re) + acc[4];
-    if (grad) {
-        const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
-        for (int f = 0; f < a->nf; ++f)
-            if (grad[f])
-                HIPCHK(hipMemcpy2D(grad[f], W * sizeof(double), L + f * a->fe + XOFF,
-                                   w->g.px * sizeof(double), W * sizeof(double), rows,
-                                   hipMemcpyDeviceToHost));
-    }
-    if (dJ_dre) *dJ_dre = g_re;
-    if (dJ_dth) *dJ_dth = acc[3];
     a->bwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
                     .count();
+    // ore = 1/Re, tk = ocv/(Re Pr); acc[4] holds the dt limits' d/dRe
+    const double g_re = acc[0] * (-C.ore * C.ore) + acc[1] * (-C.tk / C.re) + acc[4];
+    if (grad)
+        for (int f = 0; f < a->nf; ++f)
+            if (grad[f] && (rc = global_copy(w, L + f * a->fe, grad[f], true))) return rc;
+    if (dJ_dre) *dJ_dre = g_re;
+    if (dJ_dth) *dJ_dth = acc[3];
     return R3D_OK;
 }
 
@@ -731,13 +895,11 @@ int r3d_adj_timing(r3d_adj_ctx* a, double* fwd, double* bwd) {
 
 int r3d_adj_download_final(r3d_adj_ctx* a, double* const* fields) {
     if (!a || !fields) return set_err(R3D_EINVAL, "null argument");
-    r3d_hip_ctx* w = a->work;
-    const size_t W = (size_t)(w->cfg.npx + 2), rows = (size_t)(w->cfg.npy + 2) * (w->cfg.npz + 2);
     for (int f = 0; f < a->nf; ++f)
-        if (fields[f])
-            HIPCHK(hipMemcpy2D(fields[f], W * sizeof(double), a->xfinal + f * a->fe + XOFF,
-                               w->g.px * sizeof(double), W * sizeof(double), rows,
-                               hipMemcpyDeviceToHost));
+        if (fields[f]) {
+            int rc = global_copy(a->work, a->xfinal + f * a->fe, fields[f], true);
+            if (rc) return rc;
+        }
     return R3D_OK;
 }
 

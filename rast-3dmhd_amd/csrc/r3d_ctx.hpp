@@ -47,6 +47,7 @@ struct r3d_hip_ctx {
     r3d::DevScal* scal = nullptr;
     double* metric_buf = nullptr;
     double* stage_buf = nullptr;
+    double* gstage = nullptr;  // one global field, (npx+2)(npy+2)(npz+2), allocated on first use
     r3d::Metrics M{};
     r3d::Consts C{};
     bool metrics_set = false;
@@ -68,6 +69,10 @@ inline size_t field_elems(const Geo& g) { return (size_t)g.pp * (g.npz + 2) + 16
 
 // hipMalloc with the error mapped to R3D_ENOMEM; counts bytes on c.
 int dev_alloc(r3d_hip_ctx* c, double** p, size_t n);
+// Copy one padded global field (device, at its allocation base) to or from a
+// host array of (npx+2)(npy+2)(npz+2) doubles, x fastest: packed on the device
+// and moved in one transfer.  Synchronises c->stream.
+int global_copy(r3d_hip_ctx* c, double* field, double* host, bool to_host);
 // Write the periodic ghost images of c's current X (downloads read them).
 int fill_ghosts(r3d_hip_ctx* c);
 // All virtual ranks and the metrics uploaded.

@@ -57,6 +57,22 @@ __global__ void k_rank_copy(double* glob, double* loc, Geo g, int nx, int ny, in
     }
 }
 
+// Padded global field <-> dense (npx+2)(npy+2)(npz+2) array, x fastest.
+__global__ void k_global_copy(double* padded, double* dense, Geo g, int to_dense) {
+    const long W = g.npx + 2, H = g.npy + 2;
+    const long total = W * H * (g.npz + 2);
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % W);
+        const long rest = q / W;
+        const long n = g.at(i, (int)(rest % H), (int)(rest / H));
+        if (to_dense)
+            dense[q] = padded[n];
+        else
+            padded[n] = dense[q];
+    }
+}
+
 // Kernels of one run over the device functions of r3d_aux.hpp.
 __global__ __launch_bounds__(BOUND_THREADS) void k_bound(const StageArgs a, const Consts C,
                                                          const Metrics M, int partial_base) {
@@ -258,6 +274,27 @@ int launch_dt(r3d_hip_ctx* c) {
 
 }  // namespace
 
+int r3d::global_copy(r3d_hip_ctx* c, double* field, double* host, bool to_host) {
+    const r3d_hip_cfg& f = c->cfg;
+    const size_t n = (size_t)(f.npx + 2) * (f.npy + 2) * (f.npz + 2);
+    int rc;
+    if (!c->gstage && (rc = dev_alloc(c, &c->gstage, n))) return rc;
+    const dim3 grid((unsigned)std::min<size_t>(4096, (n + 255) / 256)), blk(256);
+    if (to_host) {
+        hipLaunchKernelGGL(k_global_copy, grid, blk, 0, c->stream, field, c->gstage, c->g, 1);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(host, c->gstage, n * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(c->gstage, host, n * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        hipLaunchKernelGGL(k_global_copy, grid, blk, 0, c->stream, field, c->gstage, c->g, 0);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return R3D_OK;
+}
+
 int r3d::fill_ghosts(r3d_hip_ctx* c) {
     State s;
     for (int f = 0; f < NXF; ++f) s.q[f] = c->xs[c->cur][f];
@@ -414,6 +451,7 @@ int r3d_hip_destroy(r3d_hip_ctx* c) {
     if (c->scal) (void)hipFree(c->scal);
     if (c->metric_buf) (void)hipFree(c->metric_buf);
     if (c->stage_buf) (void)hipFree(c->stage_buf);
+    if (c->gstage) (void)hipFree(c->gstage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return R3D_OK;
@@ -496,16 +534,13 @@ int r3d_hip_upload_rank(r3d_hip_ctx* c, int rank, const double* const* fields) {
 
 int r3d_hip_upload_global(r3d_hip_ctx* c, const double* const* fields) {
     if (!c || !fields) return set_err(R3D_EINVAL, "null argument");
-    const r3d_hip_cfg& f = c->cfg;
-    const size_t W = (size_t)(f.npx + 2), rows = (size_t)(f.npy + 2) * (f.npz + 2);
     for (int q = 0; q < c->nf; ++q) {
         if (!fields[q]) return set_err(R3D_EINVAL, "field %d missing", q);
-        for (int s = 0; s < 2; ++s)
-            HIPCHK(hipMemcpy2DAsync(c->xs[s][q] + XOFF, c->g.px * sizeof(double), fields[q],
-                                    W * sizeof(double), W * sizeof(double), rows,
-                                    hipMemcpyHostToDevice, c->stream));
+        for (int s = 0; s < 2; ++s) {
+            int rc = global_copy(c, c->xs[s][q], const_cast<double*>(fields[q]), false);
+            if (rc) return rc;
+        }
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
     std::fill(c->uploaded.begin(), c->uploaded.end(), 1);
     c->next_valid = false;
     return R3D_OK;
@@ -554,11 +589,8 @@ int r3d_hip_download_global(r3d_hip_ctx* c, double* const* fields) {
             continue;
         }
         double* src = isz ? c->z[fi] : c->xs[c->cur][fi];
-        HIPCHK(hipMemcpy2DAsync(fields[q], W * sizeof(double), src + XOFF,
-                                c->g.px * sizeof(double), W * sizeof(double), rows,
-                                hipMemcpyDeviceToHost, c->stream));
+        if ((rc = global_copy(c, src, fields[q], true))) return rc;
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
     return R3D_OK;
 }
 

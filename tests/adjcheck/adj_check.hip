@@ -211,4 +211,66 @@ void adj_vjp_mag(const int* dims, const double* const* X, const double* const* B
             }
 }
 
+// LMAG terms alone (cell_flux_mag_extra) over the interior.
+void adj_flux_mag_extra(const int* dims, const double* const* X, const double* const* B,
+                        const double* const* met, const double* kc, double* const* F) {
+    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4], dims[5]};
+    const Consts C = make_consts(kc);
+    for (int k = 1; k <= d.nz - 2; ++k)
+        for (int j = 1; j <= d.ny - 2; ++j)
+            for (int i = 1; i <= d.nx - 2; ++i) {
+                Stencil s;
+                MagStencil b;
+                CellMet m;
+                gather(d, X, i, j, k, s, m, met);
+                gather_mag(d, X, B, i, j, k, m, b);
+                double f[8];
+                cell_flux_mag_extra(s, b, m, C, f);
+                const long n = at(d, i, j, k);
+                for (int q = 0; q < 8; ++q) F[q][n] = f[q];
+            }
+}
+
+// Stencil entries outside vjp_mask that the transposes set nonzero, over
+// every interior cell: mode 0 cell_flux_vjp, 1 cell_flux_mag_vjp,
+// 2 cell_flux_mag_extra_vjp alone.
+long adj_mask_violations(const int* dims, const double* const* X, const double* const* B,
+                         const double* const* met, const double* kc,
+                         const double* const* lamF, int mode) {
+    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4], dims[5]};
+    const Consts C = make_consts(kc);
+    long bad = 0;
+    for (int k = 1; k <= d.nz - 2; ++k)
+        for (int j = 1; j <= d.ny - 2; ++j)
+            for (int i = 1; i <= d.nx - 2; ++i) {
+                Stencil s, A;
+                MagStencil b, AB;
+                CellMet m;
+                gather(d, X, i, j, k, s, m, met);
+                if (mode) gather_mag(d, X, B, i, j, k, m, b);
+                const long n = at(d, i, j, k);
+                double lam[8];
+                for (int q = 0; q < (mode ? 8 : 5); ++q) lam[q] = lamF[q][n];
+                double go, gt;
+                if (mode == 0) {
+                    cell_flux_vjp(s, m, C, lam, A, go, gt);
+                } else if (mode == 1) {
+                    cell_flux_mag_vjp(s, b, m, C, lam, A, AB, go, gt);
+                } else {
+                    for (int q = 0; q < NPTS; ++q) A.p[q] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                    A.rw2 = 0.0;
+                    cell_flux_mag_extra_vjp(s, b, m, C, lam, A, AB);
+                }
+                for (int q = 0; q < NPTS; ++q) {
+                    const unsigned msk = vjp_mask(q, mode != 0, mode == 2);
+                    const double v[8] = {A.p[q].ru, A.p[q].rv, A.p[q].rw, A.p[q].ro, A.p[q].tt,
+                                         mode ? AB.p[q].bx : 0.0, mode ? AB.p[q].by : 0.0,
+                                         mode ? AB.p[q].bz : 0.0};
+                    for (int f = 0; f < 8; ++f)
+                        if (!(msk & (1u << f)) && v[f] != 0.0) ++bad;
+                }
+            }
+    return bad;
+}
+
 }  // extern "C"

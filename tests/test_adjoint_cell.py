@@ -7,7 +7,10 @@ run on the CPU through tests/adjcheck/adj_check.hip:
   the adjoint's inner product matches a central finite difference of
   lam . F, field by field, and likewise for the coefficients ore and tk;
 * the same for the MHD pair cell_flux_mag / cell_flux_mag_vjp against the
-  oracle's LMAG branch, with a fully three-dimensional B.
+  oracle's LMAG branch, with a fully three-dimensional B;
+* cell_flux_mag_extra is the LMAG part of cell_flux_mag (the GPU transpose
+  of an MHD stage runs it as a kernel of its own), and every stencil entry
+  outside vjp_mask, which the tiled GPU transpose does not store, is zero.
 """
 import ctypes
 import os
@@ -46,6 +49,10 @@ def lib(tmp_path_factory):
     L.adj_vjp.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, dp, pp, pp, dp]
     L.adj_flux_mag.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, pp, dp, pp]
     L.adj_vjp_mag.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, pp, dp, pp, pp, dp]
+    L.adj_flux_mag_extra.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, pp, dp, pp]
+    L.adj_mask_violations.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, pp, dp, pp,
+                                      ctypes.c_int]
+    L.adj_mask_violations.restype = ctypes.c_long
     return L
 
 
@@ -231,3 +238,38 @@ def test_vjp_mag_matches_finite_differences(lib, msim, field):
         phi = lambda F: sum(float(np.sum(l * f)) for l, f in zip(lam, F))
         fd = (phi(rk.flux_mag(lib, P[:5], P[5:])) - phi(rk.flux_mag(lib, M[:5], M[5:]))) / (2 * eps)
         assert abs(adj - fd) <= 1e-7 * abs(fd) + 1e-12, (r, names[field], adj, fd)
+
+
+def test_lmag_terms_are_the_difference_of_the_fluxes(lib, msim):
+    for r in range(msim.nranks):
+        rk = Rank(msim, r)
+        X = [msim.field(r, n).copy() for n in X_NAMES]
+        B = [msim.field(r, n).copy() for n in B_NAMES]
+        Fm = rk.flux_mag(lib, X, B)
+        Fh = rk.flux(lib, X)
+        E = [np.zeros_like(X[0]) for _ in range(8)]
+        lib.adj_flux_mag_extra(rk.dims, _pp(X), _pp(B), _pp(rk.met),
+                               rk.k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _pp(E))
+        sl = (slice(1, -1),) * 3
+        for q in range(8):
+            want = Fm[q][sl] - (Fh[q][sl] if q < 5 else 0.0)
+            # rounding of the hydrodynamic part, relative to the whole flux
+            tol = 1e-13 * (np.abs(Fm[q][sl]).max() + 1e-300)
+            assert np.abs(E[q][sl] - want).max() <= tol, (r, q)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=["hydro", "mhd", "lmag_terms"])
+def test_transpose_is_zero_outside_the_stored_mask(lib, sim, msim, mode):
+    s = msim if mode else sim
+    rng = np.random.default_rng(77 + mode)
+    names = X_NAMES + (B_NAMES if mode else [])
+    for r in range(s.nranks):
+        rk = Rank(s, r)
+        X = [s.field(r, n).copy() for n in X_NAMES]
+        B = [s.field(r, n).copy() for n in B_NAMES] if mode else X[:3]
+        mask = _interior_mask(X[0].shape)
+        lam = [rng.uniform(-1, 1, X[0].shape) * mask for _ in range(8)]
+        bad = lib.adj_mask_violations(rk.dims, _pp(X), _pp(B), _pp(rk.met),
+                                      rk.k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      _pp(lam), mode)
+        assert bad == 0, (r, names, bad)
