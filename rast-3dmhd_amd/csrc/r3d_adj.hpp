@@ -38,9 +38,20 @@ __host__ __device__ inline void pt_offset(int p, int& di, int& dj, int& dk) {
     dk = o[p][2];
 }
 
+// A stencil point: the primaries and, in a state stencil, 1/RO and the
+// velocities derived from them (derive_point; the operations of the stage
+// kernel's derive, so the same doubles).  An adjoint stencil uses the first
+// five members only.
 struct Pt {
     double ru, rv, rw, ro, tt;
+    double ri, u, v, w;
 };
+__host__ __device__ inline void derive_point(Pt& q) {
+    q.ri = 1.0 / q.ro;
+    q.u = q.ru * q.ri;
+    q.v = q.rv * q.ri;
+    q.w = q.rw * q.ri;
+}
 
 struct Stencil {
     Pt p[NPTS];
@@ -86,10 +97,10 @@ __host__ __device__ inline void cell_flux(const Stencil& s, const CellMet& m, co
                                           double F[5]) {
     double ri[NPTS], u[NPTS], v[NPTS], w[NPTS], pr[NPTS];
     for (int q = 0; q < NPTS; ++q) {
-        ri[q] = 1.0 / s.p[q].ro;
-        u[q] = s.p[q].ru * ri[q];
-        v[q] = s.p[q].rv * ri[q];
-        w[q] = s.p[q].rw * ri[q];
+        ri[q] = s.p[q].ri;
+        u[q] = s.p[q].u;
+        v[q] = s.p[q].v;
+        w[q] = s.p[q].w;
         pr[q] = s.p[q].ro * s.p[q].tt;
     }
     const double hxd = C.hx, dxx = m.dxx, d2xx = m.d2xx, dyy = m.dyy, d2yy = m.d2yy;
@@ -205,10 +216,10 @@ __host__ __device__ inline void cell_flux_vjp(const Stencil& s, const CellMet& m
                                               double& g_tk) {
     double ri[NPTS], u[NPTS], v[NPTS], w[NPTS];
     for (int q = 0; q < NPTS; ++q) {
-        ri[q] = 1.0 / s.p[q].ro;
-        u[q] = s.p[q].ru * ri[q];
-        v[q] = s.p[q].rv * ri[q];
-        w[q] = s.p[q].rw * ri[q];
+        ri[q] = s.p[q].ri;
+        u[q] = s.p[q].u;
+        v[q] = s.p[q].v;
+        w[q] = s.p[q].w;
     }
     // adjoints of the derived point quantities
     double au[NPTS], av[NPTS], aw[NPTS], ap[NPTS];
@@ -416,15 +427,14 @@ __host__ __device__ inline void cell_flux_mag(const Stencil& s, const MagStencil
     cell_flux(s, m, C, F);
     double u[NPTS], v[NPTS], w[NPTS], bx[NPTS], by[NPTS], bz[NPTS];
     for (int q = 0; q < NPTS; ++q) {
-        const double ri = 1.0 / s.p[q].ro;
-        u[q] = s.p[q].ru * ri;
-        v[q] = s.p[q].rv * ri;
-        w[q] = s.p[q].rw * ri;
+        u[q] = s.p[q].u;
+        v[q] = s.p[q].v;
+        w[q] = s.p[q].w;
         bx[q] = b.p[q].bx;
         by[q] = b.p[q].by;
         bz[q] = b.p[q].bz;
     }
-    const double R = 1.0 / s.p[P_C].ro;
+    const double R = s.p[P_C].ri;
     const double hxd = C.hx, dxx = m.dxx, d2xx = m.d2xx, dyy = m.dyy, d2yy = m.d2yy;
     const double dzz = m.dzz, d2zz = m.d2zz;
 #define DXS(q) ((q[P_E] - q[P_W]) * hxd * dxx)
@@ -539,15 +549,14 @@ __host__ __device__ inline void cell_flux_mag_extra(const Stencil& s, const MagS
                                                     double E[8]) {
     double u[NFACE], v[NFACE], w[NFACE], bx[NFACE], by[NFACE], bz[NFACE];
     for (int q = 0; q < NFACE; ++q) {
-        const double ri = 1.0 / s.p[q].ro;
-        u[q] = s.p[q].ru * ri;
-        v[q] = s.p[q].rv * ri;
-        w[q] = s.p[q].rw * ri;
+        u[q] = s.p[q].u;
+        v[q] = s.p[q].v;
+        w[q] = s.p[q].w;
         bx[q] = b.p[q].bx;
         by[q] = b.p[q].by;
         bz[q] = b.p[q].bz;
     }
-    const double R = 1.0 / s.p[P_C].ro;
+    const double R = s.p[P_C].ri;
     const double hxd = C.hx * m.dxx, hyd = C.hy * m.dyy, hzd = C.hz * m.dzz;
     const double lx1 = C.hx * m.d2xx, lx2 = C.h2x * m.dxx * m.dxx;
     const double ly1 = C.hy * m.d2yy, ly2 = C.h2y * m.dyy * m.dyy;
@@ -604,10 +613,10 @@ __host__ __device__ inline void cell_flux_mag_extra_vjp(const Stencil& s, const 
     double au[NFACE], av[NFACE], aw[NFACE];
     for (int q = 0; q < NPTS; ++q) AB.p[q].bx = AB.p[q].by = AB.p[q].bz = 0.0;
     for (int q = 0; q < NFACE; ++q) {
-        ri[q] = 1.0 / s.p[q].ro;
-        u[q] = s.p[q].ru * ri[q];
-        v[q] = s.p[q].rv * ri[q];
-        w[q] = s.p[q].rw * ri[q];
+        ri[q] = s.p[q].ri;
+        u[q] = s.p[q].u;
+        v[q] = s.p[q].v;
+        w[q] = s.p[q].w;
         bx[q] = b.p[q].bx;
         by[q] = b.p[q].by;
         bz[q] = b.p[q].bz;

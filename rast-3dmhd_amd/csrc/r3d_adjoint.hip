@@ -15,6 +15,10 @@
 // and through the thermal and viscous dt limits; theta through the boundary
 // heat-flux conditions.
 //
+// This file is compiled with -ffp-contract=fast (the transposes need no
+// reference rounding); the objective and the dt argmin, which must round as
+// the forward does, switch contraction off.
+//
 // The states X_n, X1, X2 of every step of one segment are recomputed from
 // the segment's checkpoint by one forward pass that keeps the stage states.
 // Three adjoint buffers rotate through the stages, so that every flux
@@ -63,6 +67,7 @@ __device__ void gather_stencil(const Geo& g, const State& X, const Metrics& M, i
         pt_offset(p, di, dj, dk);
         const long n = g.at(wrapx(g, i + di), wrapy(g, j + dj), k + dk);
         s.p[p] = {X.q[QRU][n], X.q[QRV][n], X.q[QRW][n], X.q[QRO][n], X.q[QTT][n]};
+        derive_point(s.p[p]);
     }
     m.wall = k == 1 ? 1 : (k == g.npz ? 2 : 0);
     s.rw2 = m.wall == 1 ? X.q[QRW][g.at(i, j, k + 2)]
@@ -199,148 +204,297 @@ __global__ __launch_bounds__(AT) void k_flux_vjp(const Geo g, const State X, con
     block_sum(dot, red, &acc[2]);
 }
 
-// Tiled transpose: one workgroup owns a 64 x VT column tile (lane = x, wave =
-// y) and a chunk of kc planes, and marches up the chunk.  Each cell's stencil
-// transpose is added into an LDS ring of three planes (k-1, k, k+1) of the
-// tile plus a one-cell x/y halo; once plane k is done, plane k-1 has received
-// every contribution this workgroup makes to it and is added to LO with one
-// global atomic per nonzero entry (the halo cells and the chunk's edge planes
-// also receive contributions from neighbouring workgroups).  Same results as
-// k_flux_vjp up to the order of the sums.  LO must be zero on entry.
-//
-// LMAG = false: the hydrodynamic transpose (cell_flux_vjp, 19 points) of
-// lamF's first five components; true: the transpose of the LMAG terms alone
-// (cell_flux_mag_extra_vjp, face points), so that an MHD stage is the sum of
-// two launches, each small enough to stay in registers.
-constexpr int VT = 4;  // rows of the tile
-constexpr int VRW = 66, VRH = VT + 2;
+// f[k * pp] plane base (uniform over the workgroup) plus a per-lane byte offset
+__device__ __forceinline__ double ld_at(const double* f, long pp, int k, uint32_t boff) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(f + pp * (long)k) +
+                                            boff);
+}
 
-template <bool LMAG>
-__global__ __launch_bounds__(64 * VT) void k_vjp_tile(const Geo g, const State X, const Consts C,
-                                                      const Metrics M, const State LA, double ca,
-                                                      const State LB, double cb, const State LO,
-                                                      double* acc, int kc) {
-    constexpr int NF = LMAG ? 8 : 5;      // lamF components used and ring fields
-    constexpr int NP = LMAG ? NFACE : NPTS;
-    constexpr int F0 = LMAG ? QBX : QRU;  // LB passed through for fields F0..NF-1
-    constexpr int SLOT = NF * VRH * VRW;  // doubles of one ring plane
+// Tiled transpose: one workgroup of 256 threads owns a VTW x VTH column tile
+// (thread t at x = t % VTW, y = t / VTW) and a chunk of kc planes, and marches
+// up the chunk.  The tile is square so that its frame and halo, whose entries
+// other workgroups also reach and which go out as global atomics, are the
+// smallest share of its cells.
+//
+// State.  The workgroup stages planes k-1, k, k+1 of the tile plus a one-cell
+// x/y halo (periodic neighbours loaded from the cells they wrap to) in an LDS
+// ring, each staged cell with 1/RO and the velocities already derived (once
+// per cell instead of once per stencil point), and every cell reads its
+// stencil from there; plane k+2 is fetched into registers while plane k is
+// computed.
+//
+// Adjoint.  Each cell's stencil transpose is added into a second LDS ring of
+// three planes; once plane k is done, plane k-1 has received every
+// contribution this workgroup makes to it and is written out.  A cell that
+// nothing else reaches (off the tile's frame, in a chunk-interior plane, not on
+// a periodic x/y edge, which halos wrap onto, and not in a plane the wall
+// cells' extra entries reach) is stored; every other entry is added with a
+// global atomic, LO being zero on entry.  Same results as k_flux_vjp up to
+// rounding (sum order; this file's transpose kernels contract products and
+// sums into fused multiply-adds).
+//
+// MAG: per cell, the hydrodynamic transpose (cell_flux_vjp, 19 points) is
+// emitted first and the transpose of the LMAG terms (cell_flux_mag_extra_vjp,
+// face points) after it, so that the two never hold their stencils in
+// registers at the same time.
+constexpr int VTW = 16, VTH = 256 / VTW;  // tile width (x) and height (y)
+constexpr int VRW = VTW + 2, VRH = VTH + 2, VPL = VRH * VRW;
+enum { SS_RU = 0, SS_RV, SS_RW, SS_RO, SS_TT, SS_RI, SS_U, SS_V, SS_W, SS_BX, SS_BY, SS_BZ };
+
+template <bool MAG>
+__global__ __launch_bounds__(256) void k_vjp_tile(const Geo g, const State X, const Consts C,
+                                                  const Metrics M, const State LA, double ca,
+                                                  const State LB, double cb, const State LO,
+                                                  double* acc, int kc) {
+    constexpr int NF = MAG ? 8 : 5;           // primaries: lamF components, adjoint ring fields
+    constexpr int NS = MAG ? 12 : 9;          // staged state fields (SS_*)
+    constexpr int SLOT = NF * VPL, SSLOT = NS * VPL;
+    constexpr int NPC = (VPL + 255) / 256;  // staged cells per thread and plane
     __shared__ double ring[3 * SLOT];
-    __shared__ double red[VT];
-    const int ntx = (g.npx + 63) / 64, nty = (g.npy + VT - 1) / VT;
+    __shared__ double st[3 * SSLOT];
+    __shared__ double red[4];
+    const int ntx = (g.npx + VTW - 1) / VTW, nty = (g.npy + VTH - 1) / VTH;
     const int tx = blockIdx.x % ntx, ty = (blockIdx.x / ntx) % nty, tz = blockIdx.x / (ntx * nty);
-    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    const int i = tx * 64 + lane + 1, j = ty * VT + row + 1;
+    const int lx = threadIdx.x % VTW, ly = threadIdx.x / VTW;
+    const int i = tx * VTW + lx + 1, j = ty * VTH + ly + 1;
     const bool act = i <= g.npx && j <= g.npy;
     const int k0 = 1 + tz * kc, k1 = min(g.npz, k0 + kc - 1);
-    for (int e = threadIdx.x; e < 3 * SLOT; e += 64 * VT) ring[e] = 0.0;
-    auto slot = [&](int kk) { return ((kk - k0 + 1) % 3) * SLOT; };
-    // add ring plane kk to LO and clear it
+    for (int e = threadIdx.x; e < 3 * SLOT; e += 256) ring[e] = 0.0;
+    auto slot = [&](int kk) { return (kk - k0 + 1) % 3; };
+
+    // ring cell (rr, c) <-> global (i, j) = (tx*VTW + c, ty*VTH + rr), wrapped
+    auto wrap_i = [&](int gi) { return gi == 0 ? g.npx : (gi == g.npx + 1 ? 1 : gi); };
+    auto wrap_j = [&](int gj) { return gj == 0 ? g.npy : (gj == g.npy + 1 ? 1 : gj); };
+    // this thread's staged cells: byte offsets within a plane (~0u: none)
+    uint32_t poff[NPC];
+    for (int t = 0; t < NPC; ++t) {
+        const int e = threadIdx.x + 256 * t;
+        const int c = e % VRW, rr = e / VRW;
+        const int gi = tx * VTW + c, gj = ty * VTH + rr;
+        poff[t] = (e < VPL && gi <= g.npx + 1 && gj <= g.npy + 1)
+                      ? (uint32_t)(8 * (XOFF + wrap_i(gi) + g.px * (long)wrap_j(gj)))
+                      : 0xffffffffu;
+    }
+    auto fetch = [&](int kk, double (*pf)[NF]) {
+        for (int t = 0; t < NPC; ++t) {
+            uint32_t off = poff[t];
+            asm volatile("" : "+v"(off));
+            for (int f = 0; f < NF; ++f)
+                pf[t][f] = off != 0xffffffffu ? ld_at(X.q[f], g.pp, kk, off) : 1.0;
+        }
+    };
+    auto put = [&](int kk, double (*pf)[NF]) {
+        double* d = st + slot(kk) * SSLOT;
+        for (int t = 0; t < NPC; ++t) {
+            const int e = threadIdx.x + 256 * t;
+            if (e >= VPL) continue;
+            Pt q{pf[t][QRU], pf[t][QRV], pf[t][QRW], pf[t][QRO], pf[t][QTT]};
+            derive_point(q);
+            d[SS_RU * VPL + e] = q.ru;
+            d[SS_RV * VPL + e] = q.rv;
+            d[SS_RW * VPL + e] = q.rw;
+            d[SS_RO * VPL + e] = q.ro;
+            d[SS_TT * VPL + e] = q.tt;
+            d[SS_RI * VPL + e] = q.ri;
+            d[SS_U * VPL + e] = q.u;
+            d[SS_V * VPL + e] = q.v;
+            d[SS_W * VPL + e] = q.w;
+            if constexpr (MAG) {
+                d[SS_BX * VPL + e] = pf[t][QBX];
+                d[SS_BY * VPL + e] = pf[t][QBY];
+                d[SS_BZ * VPL + e] = pf[t][QBZ];
+            }
+        }
+    };
+    // write adjoint ring plane kk out and clear it
     auto flush = [&](int kk) {
-        double* r = ring + slot(kk);
-        for (int e = threadIdx.x; e < SLOT; e += 64 * VT) {
+        double* r = ring + slot(kk) * SLOT;
+        // planes 3, 4 and npz-2, npz-3 also receive the wall cells' extra
+        // entries (two and three planes in), added by atomics as they occur
+        const bool inner_plane = kk > k0 && kk < k1 && kk != 3 && kk != 4 && kk != g.npz - 2 &&
+                                 kk != g.npz - 3;
+        for (int e = threadIdx.x; e < SLOT; e += 256) {
             const double v = r[e];
             if (v == 0.0) continue;
             r[e] = 0.0;
-            const int f = e / (VRH * VRW), rc = e % (VRH * VRW);
-            int gi = tx * 64 + rc % VRW, gj = ty * VT + rc / VRW;  // 0 .. npx+1 / npy+1
+            const int f = e / VPL, rc = e % VPL;
+            const int c = rc % VRW, rr = rc / VRW;
+            const int gi = tx * VTW + c, gj = ty * VTH + rr;  // 0 .. npx+1 / npy+1
             if (gi > g.npx + 1 || gj > g.npy + 1) continue;  // past a partial tile: never set
-            if (gi == 0) gi = g.npx;
-            else if (gi == g.npx + 1) gi = 1;
-            if (gj == 0) gj = g.npy;
-            else if (gj == g.npy + 1) gj = 1;
-            atomicAdd(&LO.q[f][g.at(gi, gj, kk)], v);
+            const bool owned = inner_plane && c >= 2 && c <= VTW - 1 && rr >= 2 && rr <= VTH - 1 &&
+                               gi >= 2 && gi <= g.npx - 1 && gj >= 2 && gj <= g.npy - 1;
+            double* out = &LO.q[f][g.at(wrap_i(gi), wrap_j(gj), kk)];
+            if (owned)
+                *out = v;
+            else
+                atomicAdd(out, v);
         }
     };
-    double go = 0.0, gt = 0.0, dot = 0.0;
-    __syncthreads();
-    for (int k = k0; k <= k1; ++k) {
-        if (act) {
-            const long n = g.at(i, j, k);
-            double lam[8];
+    // add a cell's stencil entries (those vjp_mask allows) into the ring
+    auto emit = [&](int k, int np, const Stencil& A, const MagStencil* AB, bool lmag_terms) {
+#pragma unroll
+        for (int p = 0; p < NPTS; ++p) {
+            if (p >= np) break;
+            int di, dj, dk;
+            pt_offset(p, di, dj, dk);
+            const int kk = k + dk;
+            if (kk < 1 || kk > g.npz) continue;  // outer planes are not state
+            double* rp = ring + slot(kk) * SLOT + (ly + 1 + dj) * VRW + (lx + 1 + di);
+            const unsigned msk = vjp_mask(p, MAG && !lmag_terms, lmag_terms);
+            const double vals[8] = {A.p[p].ru, A.p[p].rv, A.p[p].rw, A.p[p].ro, A.p[p].tt,
+                                    AB ? AB->p[p].bx : 0.0, AB ? AB->p[p].by : 0.0,
+                                    AB ? AB->p[p].bz : 0.0};
+#pragma unroll
             for (int f = 0; f < NF; ++f)
-                lam[f] = cb * LB.q[f][n] + (ca != 0.0 ? ca * LA.q[f][n] : 0.0);
-            Stencil s, A;
+                if (msk & (1u << f) && (f < 5 || AB)) atomicAdd(rp + f * VPL, vals[f]);
+        }
+    };
+
+    // Global accesses are ordered so that no wait drains a later prefetch: the
+    // vector-memory counter completes in issue order, so each plane first
+    // issues the next plane's lamF inputs, then the state prefetch two planes
+    // ahead, and its flush atomics last; the plane's compute then waits only
+    // for lamF loads issued one plane earlier.
+    double go = 0.0, gt = 0.0, dot = 0.0;
+    double pf[NPC][NF];
+    double lbn[NF], lan[NF];  // LB, LA at (i, j) of the next plane
+    const uint32_t coff = (uint32_t)(8 * (XOFF + min(i, g.npx) + g.px * (long)min(j, g.npy)));
+    auto fetch_lam = [&](int kk) {
+        uint32_t off = coff;
+        asm volatile("" : "+v"(off));
+        for (int f = 0; f < NF; ++f) {
+            lbn[f] = ld_at(LB.q[f], g.pp, kk, off);
+            lan[f] = ca != 0.0 ? ld_at(LA.q[f], g.pp, kk, off) : 0.0;
+        }
+    };
+    // per-column metrics, loaded once
+    const int ic = min(i, g.npx), jc = min(j, g.npy);
+    const double mdxx = M.dxx[ic], md2xx = M.d2xx[ic], mdyy = M.dyy[jc], md2yy = M.d2yy[jc];
+    fetch(k0 - 1, pf);
+    put(k0 - 1, pf);
+    fetch(k0, pf);
+    put(k0, pf);
+    fetch_lam(k0);
+    fetch(k0 + 1, pf);  // k0 + 1 <= npz + 1
+    for (int k = k0; k <= k1; ++k) {
+        put(k + 1, pf);
+        __syncthreads();  // planes k-1..k+1 staged; adjoint plane k+1 cleared
+        double own[8], lam[8];
+        for (int f = 0; f < NF; ++f) {
+            own[f] = lbn[f];
+            lam[f] = cb * lbn[f] + ca * lan[f];
+        }
+        if (k + 1 <= k1) fetch_lam(k + 1);
+        if (k + 2 <= g.npz + 1 && k + 1 <= k1) fetch(k + 2, pf);
+        if (act) {
+            const double* sl[3] = {st + slot(k - 1) * SSLOT, st + slot(k) * SSLOT,
+                                   st + slot(k + 1) * SSLOT};
             CellMet m;
-            gather_stencil<NP>(g, X, M, i, j, k, s, m);
+            m.wall = k == 1 ? 1 : (k == g.npz ? 2 : 0);
             m.mag_bot = m.wall == 2 && C.mag_bot;
-            MagStencil b, AB;
-            double F[NF];
-            if constexpr (LMAG) {
-                gather_mag<NP>(g, X, i, j, k, m, b);
-                cell_flux_mag_extra(s, b, m, C, F);
-                for (int p = 0; p < NP; ++p) A.p[p] = {0.0, 0.0, 0.0, 0.0, 0.0};
-                A.rw2 = 0.0;
-                cell_flux_mag_extra_vjp(s, b, m, C, lam, A, AB);
-            } else {
-                double a, bb;
+            m.dxx = mdxx;
+            m.d2xx = md2xx;
+            m.dyy = mdyy;
+            m.d2yy = md2yy;
+            m.dzz = M.dzz[k];
+            m.d2zz = M.d2zz[k];
+            m.rk = M.rkapa[k];
+            m.dk = M.dkapa[k];
+            const double rw2 = m.wall ? X.q[QRW][g.at(i, j, m.wall == 1 ? k + 2 : k - 2)] : 0.0;
+            // hydrodynamic part
+            {
+                Stencil s, A;
+#pragma unroll
+                for (int p = 0; p < NPTS; ++p) {
+                    int di, dj, dk;
+                    pt_offset(p, di, dj, dk);
+                    const double* pl = sl[dk + 1] + (ly + 1 + dj) * VRW + (lx + 1 + di);
+                    s.p[p] = {pl[SS_RU * VPL], pl[SS_RV * VPL], pl[SS_RW * VPL], pl[SS_RO * VPL],
+                              pl[SS_TT * VPL], pl[SS_RI * VPL], pl[SS_U * VPL], pl[SS_V * VPL],
+                              pl[SS_W * VPL]};
+                }
+                s.rw2 = rw2;
+                double F[5], a, bb;
                 cell_flux(s, m, C, F);
                 cell_flux_vjp(s, m, C, lam, A, a, bb);
                 go += a;
                 gt += bb;
-            }
-            for (int f = 0; f < NF; ++f) dot += F[f] * lam[f];
-            double own[8];
-            for (int f = F0; f < NF; ++f) own[f] = LB.q[f][n];
-            if constexpr (LMAG) {
-                AB.p[P_C].bx += own[QBX];
-                AB.p[P_C].by += own[QBY];
-                AB.p[P_C].bz += own[QBZ];
-            } else {
+                for (int f = 0; f < 5; ++f) dot += F[f] * lam[f];
                 A.p[P_C].ru += own[QRU];
                 A.p[P_C].rv += own[QRV];
                 A.p[P_C].rw += own[QRW];
                 A.p[P_C].ro += own[QRO];
                 A.p[P_C].tt += own[QTT];
+                emit(k, NPTS, A, nullptr, false);
+                if (m.wall) atomicAdd(&LO.q[QRW][g.at(i, j, m.wall == 1 ? k + 2 : k - 2)], A.rw2);
             }
+            // LMAG terms; the empty asm keeps the compiler from hoisting their
+            // stencil reads above the hydrodynamic part (the two stencils
+            // would then be live together and spill)
+            if constexpr (MAG) {
+                asm volatile("" ::: "memory");
+                Stencil s, A;
+                MagStencil b, AB;
 #pragma unroll
-            for (int p = 0; p < NP; ++p) {
-                int di, dj, dk;
-                pt_offset(p, di, dj, dk);
-                const int kk = k + dk;
-                if (kk < 1 || kk > g.npz) continue;  // outer planes are not state
-                double* r = ring + slot(kk) + (row + 1 + dj) * VRW + (lane + 1 + di);
-                const unsigned msk = vjp_mask(p, false, LMAG);
-                const double vals[8] = {A.p[p].ru, A.p[p].rv, A.p[p].rw, A.p[p].ro, A.p[p].tt,
-                                        LMAG ? AB.p[p].bx : 0.0, LMAG ? AB.p[p].by : 0.0,
-                                        LMAG ? AB.p[p].bz : 0.0};
-#pragma unroll
-                for (int f = 0; f < NF; ++f)
-                    if (msk & (1u << f)) atomicAdd(r + f * VRH * VRW, vals[f]);
-            }
-            if (m.wall && (!LMAG || m.wall == 1 || m.mag_bot))
-                atomicAdd(&LO.q[QRW][g.at(i, j, m.wall == 1 ? k + 2 : k - 2)], A.rw2);
-            if (LMAG && (m.wall == 1 || m.mag_bot)) {
-                const int dir = m.wall == 1 ? 1 : -1;
-                const long n2 = g.at(i, j, k + 2 * dir), n3 = g.at(i, j, k + 3 * dir);
-                atomicAdd(&LO.q[QRO][n2], AB.ro2);
-                atomicAdd(&LO.q[QBX][n2], AB.bx2);
-                atomicAdd(&LO.q[QBY][n2], AB.by2);
-                atomicAdd(&LO.q[QBX][n3], AB.bx3);
-                atomicAdd(&LO.q[QBY][n3], AB.by3);
+                for (int p = 0; p < NFACE; ++p) {
+                    int di, dj, dk;
+                    pt_offset(p, di, dj, dk);
+                    const double* pl = sl[dk + 1] + (ly + 1 + dj) * VRW + (lx + 1 + di);
+                    s.p[p].ri = pl[SS_RI * VPL];
+                    s.p[p].u = pl[SS_U * VPL];
+                    s.p[p].v = pl[SS_V * VPL];
+                    s.p[p].w = pl[SS_W * VPL];
+                    b.p[p] = {pl[SS_BX * VPL], pl[SS_BY * VPL], pl[SS_BZ * VPL]};
+                    A.p[p] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                }
+                s.rw2 = rw2;
+                A.rw2 = 0.0;
+                b.ro2 = b.bx2 = b.by2 = b.bx3 = b.by3 = 0.0;
+                if (m.wall == 1 || m.mag_bot) {
+                    const int dir = m.wall == 1 ? 1 : -1;
+                    const long n2 = g.at(i, j, k + 2 * dir), n3 = g.at(i, j, k + 3 * dir);
+                    b.ro2 = X.q[QRO][n2];
+                    b.bx2 = X.q[QBX][n2];
+                    b.by2 = X.q[QBY][n2];
+                    b.bx3 = X.q[QBX][n3];
+                    b.by3 = X.q[QBY][n3];
+                }
+                double E[8];
+                cell_flux_mag_extra(s, b, m, C, E);
+                cell_flux_mag_extra_vjp(s, b, m, C, lam, A, AB);
+                for (int f = 0; f < 8; ++f) dot += E[f] * lam[f];
+                AB.p[P_C].bx += own[QBX];
+                AB.p[P_C].by += own[QBY];
+                AB.p[P_C].bz += own[QBZ];
+                emit(k, NFACE, A, &AB, true);
+                if (m.wall == 1 || m.mag_bot) {
+                    const int dir = m.wall == 1 ? 1 : -1;
+                    const long n2 = g.at(i, j, k + 2 * dir), n3 = g.at(i, j, k + 3 * dir);
+                    atomicAdd(&LO.q[QRW][n2], A.rw2);
+                    atomicAdd(&LO.q[QRO][n2], AB.ro2);
+                    atomicAdd(&LO.q[QBX][n2], AB.bx2);
+                    atomicAdd(&LO.q[QBY][n2], AB.by2);
+                    atomicAdd(&LO.q[QBX][n3], AB.bx3);
+                    atomicAdd(&LO.q[QBY][n3], AB.by3);
+                }
             }
         }
-        __syncthreads();
-        if (k - 1 >= 1) flush(k - 1);  // complete for this workgroup
-        __syncthreads();
+        __syncthreads();  // plane k done: k-1 complete; state plane k-1 free
+        if (k - 1 >= 1) flush(k - 1);
     }
+    __syncthreads();
     flush(k1);
     if (k1 + 1 <= g.npz) flush(k1 + 1);
     // block sums
     auto bsum = [&](double v, double* out) {
         v = wave_sum(v);
-        if (lane == 0) red[row] = v;
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int w = 0; w < VT; ++w) t += red[w];
-            atomicAdd(out, t);
-        }
+        if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
         __syncthreads();
     };
-    if (!LMAG) {
-        bsum(go, &acc[0]);
-        bsum(gt, &acc[1]);
-    }
+    bsum(go, &acc[0]);
+    bsum(gt, &acc[1]);
     bsum(dot, &acc[2]);
 }
 
@@ -412,6 +566,7 @@ __global__ __launch_bounds__(AT) void k_bcon_adj(const Geo g, const State X, con
 // gradient: J = sum vol (ru^2+rv^2+rw^2)/(2 ro), vol = 1/(dxx dyy dzz).
 __global__ __launch_bounds__(AT) void k_ke(const Geo g, const State X, const Metrics M,
                                            const State LX, int want_grad, double* part) {
+#pragma clang fp contract(off)
     __shared__ double red[AT / 64];
     double sum = 0.0;
     const long total = (long)g.npx * g.npy * g.npz;
@@ -446,6 +601,9 @@ __global__ __launch_bounds__(AT) void k_ke(const Geo g, const State X, const Met
 // cell * 4 + limit, ties to the smaller key.
 __global__ __launch_bounds__(AT) void k_dt_argmin(const Geo g, const State X, const Consts C,
                                                   const Metrics M, double* bval, long* bkey) {
+    // the limits must round as the forward's dt reduction does, so that the
+    // minimum falls on the same cell and limit
+#pragma clang fp contract(off)
     __shared__ double rv[AT / 64];
     __shared__ long rk[AT / 64];
     double best = 1e300;
@@ -661,20 +819,23 @@ int vjp(r3d_adj_ctx* a, const double* x, const double* la, double ca, const doub
     const State sb = state_of(a, (double*)lb), so = state_of(a, lo);
     const Geo& g = w->g;
     if (a->tiled) {
-        // chunks of >= 8 planes, about 2048 workgroups in all
-        const long tiles = (long)((g.npx + 63) / 64) * ((g.npy + VT - 1) / VT);
-        long nkz = std::max<long>(1, std::min<long>(g.npz / 8, (2048 + tiles - 1) / tiles));
-        const int kc = (int)((g.npz + nkz - 1) / nkz);
-        nkz = (g.npz + kc - 1) / kc;
-        const dim3 grid((unsigned)(tiles * nkz)), blk(64 * VT);
-        hipLaunchKernelGGL(k_vjp_tile<false>, grid, blk, 0, w->stream, g, sx, w->C, w->M, sa,
-                           la ? ca : 0.0, sb, cb, so, a->acc, kc);
+        // chunks of >= 8 planes, at least about 512 workgroups in all
+        auto grid_of = [&](int& kc) {
+            const long tiles = (long)((g.npx + VTW - 1) / VTW) * ((g.npy + VTH - 1) / VTH);
+            long nkz = std::max<long>(1, std::min<long>(g.npz / 8, (512 + tiles - 1) / tiles));
+            kc = (int)((g.npz + nkz - 1) / nkz);
+            nkz = (g.npz + kc - 1) / kc;
+            return dim3((unsigned)(tiles * nkz));
+        };
+        int kc;
+        const dim3 grid = grid_of(kc);
+        if (a->nf == 8)
+            hipLaunchKernelGGL(k_vjp_tile<true>, grid, dim3(256), 0, w->stream, g, sx, w->C, w->M,
+                               sa, la ? ca : 0.0, sb, cb, so, a->acc, kc);
+        else
+            hipLaunchKernelGGL(k_vjp_tile<false>, grid, dim3(256), 0, w->stream, g, sx, w->C,
+                               w->M, sa, la ? ca : 0.0, sb, cb, so, a->acc, kc);
         HIPCHK(hipGetLastError());
-        if (a->nf == 8) {
-            hipLaunchKernelGGL(k_vjp_tile<true>, grid, blk, 0, w->stream, g, sx, w->C, w->M, sa,
-                               la ? ca : 0.0, sb, cb, so, a->acc, kc);
-            HIPCHK(hipGetLastError());
-        }
         return R3D_OK;
     }
     const dim3 grid(blocks_for(w->g)), blk(AT);

@@ -24,6 +24,7 @@ void gather(const Dims& d, const double* const* X, int i, int j, int k, Stencil&
         pt_offset(q, di, dj, dk);
         const long n = at(d, i + di, j + dj, k + dk);
         s.p[q] = {X[0][n], X[1][n], X[2][n], X[3][n], X[4][n]};
+        derive_point(s.p[q]);
     }
     m.wall = (d.top && k == 1) ? 1 : ((d.bot && k == d.nz - 2) ? 2 : 0);
     m.mag_bot = m.wall == 2 && d.mag_bot;
