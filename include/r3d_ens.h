@@ -40,7 +40,7 @@ int r3d_ens_download_rank(r3d_ens_ctx* e, int member, int rank, double* const* f
 int r3d_ens_set_scalars(r3d_ens_ctx* e, int member, const r3d_hip_scalars* s);
 int r3d_ens_get_scalars(r3d_ens_ctx* e, int member, r3d_hip_scalars* s);
 
-/* Tile shape: ty rows per tile (4 or 8 for hydro, 5 for MHD) and kc planes
+/* Tile shape: ty rows per tile (4, 8 or 9 for hydro, 5 for MHD) and kc planes
  * per z chunk (8..128; 0 = per-member default).  ntiles = tiles per stage
  * launch over all members. */
 int r3d_ens_set_tiles(r3d_ens_ctx* e, int ty, int kc);

@@ -14,7 +14,13 @@
 
 namespace r3d {
 
-constexpr int TILE_Y = 8;  // rows per hydro stage tile (waves 1..8 compute fluxes)
+constexpr int TILE_Y = 9;  // rows per hydro stage tile (waves 1..9 compute fluxes)
+// Hydro tile rows for a grid npy cells high: 9 (12 waves, every wave slot of a
+// CU at 3 per SIMD) unless 8 leaves fewer rows of the last tile idle.
+inline int hydro_tile_rows(int npy) {
+    const int w9 = (npy + 8) / 9 * 9 - npy, w8 = (npy + 7) / 8 * 8 - npy;
+    return w8 < w9 ? 8 : TILE_Y;
+}
 // MHD tiles: 8 waves per workgroup (two per SIMD), so that the larger MHD
 // working set gets 256 VGPRs per lane instead of 168
 constexpr int TILE_Y_MAG = 5;

@@ -193,7 +193,8 @@ int enqueue_step(r3d_ens_ctx* e, int cur) {
     for (int s = 0; s < 3; ++s) {
         if (e->lmag) launch_stages_ty<TILE_Y_MAG, true>(e, s, cur);
         else if (e->tile.ty == 4) launch_stages_ty<4, false>(e, s, cur);
-        else launch_stages_ty<8, false>(e, s, cur);
+        else if (e->tile.ty == 8) launch_stages_ty<8, false>(e, s, cur);
+        else launch_stages_ty<9, false>(e, s, cur);
         HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_bound_ens, dim3(e->nbb_max, 2, n), dim3(BOUND_THREADS), 0, e->stream,
                            e->d_desc, cur, s);
@@ -363,7 +364,7 @@ int r3d_ens_get_scalars(r3d_ens_ctx* e, int m, r3d_hip_scalars* s) {
 
 int r3d_ens_set_tiles(r3d_ens_ctx* e, int ty, int kc) {
     if (!e) return set_err(R3D_EINVAL, "null ensemble");
-    const bool ok_ty = e->lmag ? ty == TILE_Y_MAG : (ty == 4 || ty == 8);
+    const bool ok_ty = e->lmag ? ty == TILE_Y_MAG : (ty == 4 || ty == 8 || ty == 9);
     if (!ok_ty) return set_err(R3D_EINVAL, "tile rows %d not built for this ensemble", ty);
     if (kc != 0 && (kc < 8 || kc > 128)) return set_err(R3D_EINVAL, "kc %d outside 8..128", kc);
     e->tile = {ty, kc};
@@ -472,7 +473,7 @@ int r3d_ens_autotune(r3d_ens_ctx* e, int trial_steps, double* best_ms) {
     if (e->lmag) {
         for (int kc : {8, 16, 32, 64}) cand.push_back({TILE_Y_MAG, kc});
     } else {
-        for (int ty : {4, 8})
+        for (int ty : {4, 8, 9})
             for (int kc : {8, 16, 32, 64}) cand.push_back({ty, kc});
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;

@@ -10,6 +10,7 @@
 
 #define R3D_KNS r3d_fma
 #include "r3d_stage.hpp"
+#include "r3d_ctx.hpp"
 
 namespace r3d {
 
@@ -26,11 +27,16 @@ hipError_t launch_stage_fma(const StageArgs& a, const Consts& C, const Metrics& 
     else if (a.stage == 1) R3D_FMA(TY, 1, 2, MG);                     \
     else R3D_FMA(TY, 2, 2, MG);
     if (mag) {
-        if (ty != 5) return hipErrorInvalidValue;
-        R3D_FMA_STAGES(5, true)
+        if (ty != TILE_Y_MAG) return hipErrorInvalidValue;
+        R3D_FMA_STAGES(TILE_Y_MAG, true)
     } else {
-        if (ty != 8) return hipErrorInvalidValue;
-        R3D_FMA_STAGES(8, false)
+        if (ty == 8) {
+            R3D_FMA_STAGES(8, false)
+        } else if (ty == TILE_Y) {
+            R3D_FMA_STAGES(TILE_Y, false)
+        } else {
+            return hipErrorInvalidValue;
+        }
     }
 #undef R3D_FMA_STAGES
 #undef R3D_FMA

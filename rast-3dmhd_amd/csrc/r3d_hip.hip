@@ -229,6 +229,7 @@ StageArgs stage_args(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
 int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     const StageArgs a = stage_args(c, stage, zmode, want_dt);
     const dim3 blk((c->ty + 3) * 64);
+    hipError_t fma_err = hipSuccess;
     int rc = timed(c, 0, [&] {
 #define R3D_STAGE(TY, S, Z, MG) \
     hipLaunchKernelGGL((k_stage<TY, S, Z, MG>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
@@ -239,16 +240,23 @@ int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     else if (stage == 1) R3D_STAGE(TY, 1, 2, MG);               \
     else R3D_STAGE(TY, 2, 2, MG);
         if (c->cfg.variant == 1) {
-            (void)launch_stage_fma(a, c->C, c->M, c->ty, c->cfg.lmag != 0, c->grid, c->stream);
+            fma_err = launch_stage_fma(a, c->C, c->M, c->ty, c->cfg.lmag != 0, c->grid,
+                                       c->stream);
         } else if (c->cfg.lmag) {
             R3D_STAGES(TILE_Y_MAG, true)
         } else {
-            R3D_STAGES(TILE_Y, false)
+            if (c->ty == 8) {
+                R3D_STAGES(8, false)
+            } else {
+                R3D_STAGES(TILE_Y, false)
+            }
         }
 #undef R3D_STAGES
 #undef R3D_STAGE
     });
     if (rc) return rc;
+    if (fma_err != hipSuccess)
+        return set_err(R3D_EHIP, "FMA stage launch: %s", hipGetErrorString(fma_err));
     rc = timed(c, 1, [&] {
         hipLaunchKernelGGL(k_bound, dim3(c->nbb, 2), dim3(BOUND_THREADS), 0, c->stream, a, c->C,
                            c->M, c->nblocks);
@@ -381,7 +389,7 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     fill_consts(c);
 
     // stage tiling: 64 x ty columns, z chunks sized for >= ~8 blocks per CU
-    c->ty = f.lmag ? TILE_Y_MAG : TILE_Y;
+    c->ty = f.lmag ? TILE_Y_MAG : hydro_tile_rows(f.npy);
     c->ntx = (f.npx + 63) / 64;
     c->nty = (f.npy + c->ty - 1) / c->ty;
     c->kc = 8;

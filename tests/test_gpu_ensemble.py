@@ -59,8 +59,8 @@ SMALL = [
 ]
 
 
-@pytest.mark.parametrize("tiles", [(8, 0), (4, 16), (8, 64), (4, 8)],
-                         ids=["ty8_default", "ty4_kc16", "ty8_kc64", "ty4_kc8"])
+@pytest.mark.parametrize("tiles", [(9, 0), (4, 16), (8, 64), (4, 8), (9, 32)],
+                         ids=["ty9_default", "ty4_kc16", "ty8_kc64", "ty4_kc8", "ty9_kc32"])
 def test_ensemble_members_match_single_runs(tiles):
     say(f"[gpu] ensemble of {len(SMALL)} uneven members, tiles {tiles}")
     want = [single_run(a, 6) for a in SMALL]
@@ -84,7 +84,7 @@ def test_autotune_restores_state_and_keeps_results():
     ms = ens.autotune(2)
     ty, kc, nt = ens.tiles()
     say(f"[gpu]   chose ty={ty} kc={kc} ({nt} tiles), {ms:.3f} ms/step")
-    assert ty in (4, 8) and kc in (8, 16, 32, 64) and ms > 0
+    assert ty in (4, 8, 9) and kc in (8, 16, 32, 64) and ms > 0
     ens.step(3)
     for m in range(len(SMALL)):
         compare_member(ens, m, *want[m], "after autotune")

@@ -24,10 +24,12 @@ def say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("case", ["hydro", "mhd"])
+@pytest.mark.parametrize("case", ["hydro", "hydro_9_row_tiles", "mhd"])
 def test_fma_variant_error_after_100_steps(case):
     args = ["--npx", "64", "--npy", "64", "--npz", "128", "--npey", "1", "--npez", "2"]
     names = PRIM
+    if case == "hydro_9_row_tiles":  # npy = 72: 9-row stage tiles
+        args = ["--npx", "48", "--npy", "72", "--npz", "96", "--npey", "2", "--npez", "2"]
     if case == "mhd":
         args = ["--npx", "32", "--npy", "32", "--npz", "64", "--npey", "2", "--npez", "2",
                 "--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2",
