@@ -45,6 +45,13 @@ int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J);
  * global arrays as X0; ghost entries zero), dJ/dRe, dJ/dtheta.  NULL outputs
  * are skipped. */
 int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double* dJ_dtheta);
+/* dJ/d(tb) of the last backward: the sensitivity to the fixed bottom
+ * temperature of the default bottom condition (izcon = 0), the wall value
+ * the boundary pass imposes (0 with a bottom heat-flux condition).  tb is
+ * the static profile's bottom temperature, a function of theta: with dJ_dtheta
+ * (theta in the heat-flux conditions, tb and X0 held) and grad (X0) it gives
+ * the derivative through the initial profile as well. */
+int r3d_adj_dJ_dtb(r3d_adj_ctx* a, double* dJ_dtb);
 /* dt > 0: every step of later forward runs takes this dt instead of the CFL
  * step, and J is then a smooth function of X0, Re and theta (the CFL step is
  * a minimum over cells and limits, differentiable only where one of them

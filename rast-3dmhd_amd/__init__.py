@@ -113,6 +113,7 @@ def load():
                              ctypes.POINTER(ctypes.c_double)],
         "r3d_adj_timing": [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
         "r3d_adj_set_fixed_dt": [vp, ctypes.c_double],
+        "r3d_adj_dJ_dtb": [vp, ctypes.POINTER(ctypes.c_double)],
         "r3d_adj_download_final": [vp, ctypes.POINTER(dp)],
     }
     for name, args in sigs.items():
@@ -335,10 +336,12 @@ class Ensemble:
         return b.value
 
 
-def cfg_with(cfg, re=None, theta=None):
-    """A copy of cfg with the Reynolds number and/or theta replaced, the
-    derived coefficients recomputed as the reference does (src/params.cpp:
-    ore = 1/Re, repr = Re*Pr)."""
+def cfg_with(cfg, re=None, theta=None, tb=None):
+    """A copy of cfg with the Reynolds number, theta and/or the bottom
+    temperature tb replaced, the derived coefficients recomputed as the
+    reference does (src/params.cpp: ore = 1/Re, repr = Re*Pr).  theta here is
+    the heat-flux parameter of the boundary conditions only: the static
+    profile (and tb) built from theta come from the host setup."""
     c = Cfg()
     ctypes.pointer(c)[0] = cfg
     if re is not None:
@@ -348,6 +351,8 @@ def cfg_with(cfg, re=None, theta=None):
         c.repr = c.re * pr
     if theta is not None:
         c.theta = float(theta)
+    if tb is not None:
+        c.tb = float(tb)
     return c
 
 
@@ -407,6 +412,12 @@ class Adjoint:
         """dt > 0: later forward runs step with this dt instead of the CFL
         step (J becomes smooth in X0, Re, theta); 0: back to the CFL step."""
         _check(load().r3d_adj_set_fixed_dt(self.h, float(dt)))
+
+    def dJ_dtb(self):
+        """dJ/d(tb) of the last backward (the fixed bottom temperature)."""
+        v = ctypes.c_double()
+        _check(load().r3d_adj_dJ_dtb(self.h, ctypes.byref(v)))
+        return v.value
 
     def timing(self):
         f, b = ctypes.c_double(), ctypes.c_double()

@@ -1,21 +1,41 @@
 """torch.autograd binding of the discrete adjoint (include/r3d_adj.h).
 
-KineticEnergyRun(args, nsteps) is the differentiable map
-    (tt0, Re, theta) -> J = final kinetic energy of a hydrodynamic run,
-where tt0 is the interior initial temperature (npz, npy, npx) replacing that
-of the run's initial state (built by the host setup from the reference CLI
-flags `args`; hydrodynamic or MHD).  The forward is the plain bit-exact time step on the GPU; the
-backward is the hand-written adjoint.  Composing tt0 = T_static * (1 + delta)
-in torch gives the gradient with respect to the perturbation field delta.
+KineticEnergyRun(args, nsteps) holds a run built by the host setup from the
+reference CLI flags `args` (hydrodynamic or MHD) and offers two
+differentiable maps to J = the final kinetic energy:
+
+* prob(tt0, Re, theta): tt0 is the interior initial temperature
+  (npz, npy, npx) replacing that of the run's initial state; theta enters
+  through the heat-flux boundary conditions only (the density, the field and
+  the bottom temperature stay those of the run's setup).  Composing
+  tt0 = T_static * (1 + delta) in torch gives the gradient with respect to
+  the perturbation field delta.
+* prob.static_state(theta) and prob.kinetic_energy(fields, Re, theta, tb):
+  the static profile (rho, T, B and the bottom temperature tb) as a function
+  of theta, and J of any initial state.  Composed as
+      st = prob.static_state(theta)
+      J = prob.kinetic_energy({"ro": st["ro"], "tt": st["tt"] * (1 + delta), ...},
+                              re, theta, st["tb"])
+  they give the total derivative with respect to theta, through the initial
+  profile as well as the boundary conditions.
+
+The forward is the plain bit-exact time step on the GPU; the backward is the
+hand-written adjoint.  The static profile's theta derivative is the central
+difference of the host setup (a smooth ODE solution; step 1e-6 theta).
+theta also sets the conductivity profile when the conductivity jump is on
+(--pzp != 0); that dependence is not differentiated (the metrics are held
+fixed), so the total derivative is complete for --pzp 0 only.
 """
 import numpy as np
 import torch
 
-from . import Adjoint, PRIMARIES, Run, cfg_with
+from . import Adjoint, Run, cfg_with
 
 
 class KineticEnergyRun:
-    def __init__(self, args, nsteps, checkpoint_every=20):
+    def __init__(self, args, nsteps, checkpoint_every=20, fixed_dt=0.0):
+        self.args = list(args)
+        self.fixed_dt = float(fixed_dt)  # > 0: every step takes this dt
         self.run = Run(args)
         self.base = dict(self.run.state())
         self.metrics = self.run.metrics()
@@ -31,6 +51,7 @@ class KineticEnergyRun:
             if self._adj is not None:
                 self._adj.close()
             self._adj = Adjoint(cfg_with(self.cfg, re=re, theta=theta), self.metrics, self.every)
+            self._adj.set_fixed_dt(self.fixed_dt)
             self._key = key
         return self._adj
 
@@ -40,6 +61,35 @@ class KineticEnergyRun:
 
     def __call__(self, tt0, re, theta):
         return _KineticEnergy.apply(tt0, re, theta, self)
+
+    def static_state(self, theta):
+        """Interior static fields of the setup at this theta (no perturbation)
+        as tensors differentiable in theta: "ro", "tt", "bx", "by", "bz" (zero
+        without a field) and the bottom temperature "tb" (0-d)."""
+        if np.any(np.asarray(self.metrics["rkapa"]) != 1.0):
+            raise NotImplementedError(
+                "static_state: the conductivity profile (--pzp != 0) depends on theta and "
+                "is not differentiated")
+        out = _StaticState.apply(theta, self)
+        return dict(zip(("ro", "tt", "bx", "by", "bz", "tb"), out))
+
+    def static_numpy(self, theta):
+        run = Run(self.args + ["--theta", repr(float(theta)), "--ampt", "0"])
+        st = run.state()
+        sl = (slice(1, -1),) * 3
+        zero = np.zeros_like(st["ro"][sl])
+        arrs = [st["ro"][sl].copy(), st["tt"][sl].copy()] + [
+            st[n][sl].copy() if self.cfg.lmag else zero for n in ("bx", "by", "bz")]
+        return arrs, float(run.cfg().tb)
+
+    def kinetic_energy(self, fields, re, theta, tb):
+        """J of the initial state whose interior density, temperature and (with
+        LMAG) field are the tensors fields["ro"], fields["tt"], fields["bx"..
+        "bz"] (the velocities are the setup's), with Re, theta (heat-flux
+        conditions) and the fixed bottom temperature tb."""
+        zero = torch.zeros_like(fields["tt"])
+        b = [fields.get(n, zero) for n in ("bx", "by", "bz")]
+        return _KineticEnergyState.apply(fields["ro"], fields["tt"], *b, re, theta, tb, self)
 
 
 class _KineticEnergy(torch.autograd.Function):
@@ -60,3 +110,63 @@ class _KineticEnergy(torch.autograd.Function):
         g = torch.from_numpy(grads["tt"][1:-1, 1:-1, 1:-1].copy()).to(ctx.dev)
         return (gJ * g, gJ * torch.tensor(dre, dtype=torch.float64, device=ctx.dev),
                 gJ * torch.tensor(dth, dtype=torch.float64, device=ctx.dev), None)
+
+
+class _StaticState(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, prob):
+        arrs, tb = prob.static_numpy(float(theta))
+        ctx.prob, ctx.theta, ctx.dev = prob, float(theta), theta.device
+        dev = theta.device
+        return tuple(torch.from_numpy(a).to(dev) for a in arrs) + (
+            torch.tensor(tb, dtype=torch.float64, device=dev),)
+
+    @staticmethod
+    def backward(ctx, *g):
+        th = ctx.theta
+        h = 1e-6 * abs(th) if th != 0.0 else 1e-6
+        ap, tbp = ctx.prob.static_numpy(th + h)
+        am, tbm = ctx.prob.static_numpy(th - h)
+        total = 0.0
+        for gi, p_, m_ in zip(g[:5], ap, am):
+            if gi is not None:
+                total += float(torch.sum(gi.cpu() * torch.from_numpy((p_ - m_) / (2 * h))))
+        if g[5] is not None:
+            total += float(g[5]) * (tbp - tbm) / (2 * h)
+        return torch.tensor(total, dtype=torch.float64, device=ctx.dev), None
+
+
+class _KineticEnergyState(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, ro0, tt0, bx0, by0, bz0, re, theta, tb, prob):
+        cfg = cfg_with(prob.cfg, re=float(re), theta=float(theta), tb=float(tb))
+        adj = Adjoint(cfg, prob.metrics, prob.every)
+        adj.set_fixed_dt(prob.fixed_dt)
+        fields = {k: v.copy() for k, v in prob.base.items()}
+        sl = (slice(1, -1),) * 3
+        ins = {"ro": ro0, "tt": tt0, "bx": bx0, "by": by0, "bz": bz0}
+        for n in adj.names:
+            if n in ins:
+                fields[n][sl] = ins[n].detach().to("cpu", torch.float64).numpy()
+        adj.set_initial(fields)
+        J = adj.forward(prob.nsteps)
+        ctx.adj, ctx.dev = adj, tt0.device
+        return torch.tensor(J, dtype=torch.float64, device=tt0.device)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        adj = ctx.adj
+        grads, dre, dth = adj.backward()
+        dtb = adj.dJ_dtb()
+        adj.close()
+        sl = (slice(1, -1),) * 3
+
+        def g(n):
+            if n not in grads:
+                return None
+            return gJ * torch.from_numpy(grads[n][sl].copy()).to(ctx.dev)
+
+        def sc(v):
+            return gJ * torch.tensor(v, dtype=torch.float64, device=ctx.dev)
+
+        return (g("ro"), g("tt"), g("bx"), g("by"), g("bz"), sc(dre), sc(dth), sc(dtb), None)

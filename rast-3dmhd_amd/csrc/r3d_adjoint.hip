@@ -499,11 +499,12 @@ __global__ __launch_bounds__(256) void k_vjp_tile(const Geo g, const State X, co
 }
 
 // LX <- B^T LX on the two wall planes, with X the post-boundary state;
-// acc[3] += d/d(theta).  grid = (ceil(npx*npy / AT), 2 sides).
+// acc[3] += d/d(theta), acc[5] += d/d(tb) (the fixed bottom temperature).
+// grid = (ceil(npx*npy / AT), 2 sides).
 __global__ __launch_bounds__(AT) void k_bcon_adj(const Geo g, const State X, const Consts C,
                                                  const Metrics M, const State LX, double* acc) {
     __shared__ double red[AT / 64];
-    double gth = 0.0;
+    double gth = 0.0, gtb = 0.0;
     const int side = blockIdx.y;
     const int q = blockIdx.x * AT + threadIdx.x;
     if (q < g.npx * g.npy) {
@@ -547,7 +548,8 @@ __global__ __launch_bounds__(AT) void k_bcon_adj(const Geo g, const State X, con
                     break;
             }
         } else if (C.izcon == 0) {
-            fixed = true;
+            fixed = true;  // T = tb
+            gtb = aT;
         } else {
             extrap = true;
             dth = C.c23 * M.dz_bot / M.rkapa[kw];
@@ -560,6 +562,7 @@ __global__ __launch_bounds__(AT) void k_bcon_adj(const Geo g, const State X, con
         if (fixed || extrap) LX.q[QTT][n] = 0.0;
     }
     block_sum(gth, red, &acc[3]);
+    block_sum(gtb, red, &acc[5]);
 }
 
 // Objective per block (deterministic host sum) and, if LX is given, its
@@ -740,7 +743,8 @@ struct r3d_adj_ctx {
     std::vector<double> segdt;
     double *x0 = nullptr, *xfinal = nullptr;
     double* lam[3] = {nullptr, nullptr, nullptr};  // rotating adjoint buffers
-    double* acc = nullptr;          // g_ore, g_tk, <F, lamF>, g_theta, dt's d/dRe
+    double* acc = nullptr;          // g_ore, g_tk, <F, lamF>, g_theta, dt's d/dRe, g_tb
+    double dJ_dtb = 0.0;            // of the last backward
     double* part = nullptr;         // k_ke block sums
     double* bval = nullptr;
     long* bkey = nullptr;
@@ -896,7 +900,7 @@ int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** o
     double** bufs[] = {&a->x0, &a->xfinal, &a->lam[0], &a->lam[1], &a->lam[2]};
     for (double** b : bufs)
         if ((rc = alloc_state(a, b))) return fail(rc);
-    if (hipMalloc((void**)&a->acc, 5 * sizeof(double)) != hipSuccess ||
+    if (hipMalloc((void**)&a->acc, 6 * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&a->part, ADJ_BLOCKS * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&a->bval, ADJ_BLOCKS * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&a->bkey, ADJ_BLOCKS * sizeof(long)) != hipSuccess)
@@ -988,7 +992,7 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
     double Jdummy;
     HIPCHK(hipMemsetAsync(L, 0, a->fe * a->nf * sizeof(double), w->stream));
     if ((rc = objective(a, a->xfinal, L, &Jdummy))) return rc;  // L = dJ/dX_N (B: 0)
-    HIPCHK(hipMemsetAsync(a->acc, 0, 5 * sizeof(double), w->stream));
+    HIPCHK(hipMemsetAsync(a->acc, 0, 6 * sizeof(double), w->stream));
     const double g1 = C.gam[0], g2 = C.gam[1], g3 = C.gam[2];
     const double z1 = C.zeta[0], z2 = C.zeta[1];
     const int N = a->nsteps, nseg = (N + a->K - 1) / a->K;
@@ -1024,7 +1028,7 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
             if (C.dtfix <= 0.0 && (rc = dt_adjoint(a, xn, dt, L))) return rc;
         }
     }
-    double acc[5];
+    double acc[6];
     HIPCHK(hipMemcpyAsync(acc, a->acc, sizeof acc, hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
     a->bwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
@@ -1036,6 +1040,7 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
             if (grad[f] && (rc = global_copy(w, L + f * a->fe, grad[f], true))) return rc;
     if (dJ_dre) *dJ_dre = g_re;
     if (dJ_dth) *dJ_dth = acc[3];
+    a->dJ_dtb = acc[5];
     return R3D_OK;
 }
 
@@ -1044,6 +1049,13 @@ int r3d_adj_set_fixed_dt(r3d_adj_ctx* a, double dt) {
     if (!(dt >= 0.0) || !std::isfinite(dt)) return set_err(R3D_EINVAL, "fixed dt %g", dt);
     a->work->C.dtfix = dt;
     a->nsteps = 0;  // a forward run under the new rule is needed before backward
+    return R3D_OK;
+}
+
+int r3d_adj_dJ_dtb(r3d_adj_ctx* a, double* dJ_dtb) {
+    if (!a || !dJ_dtb) return set_err(R3D_EINVAL, "null argument");
+    if (a->nsteps < 1) return set_err(R3D_ESTATE, "no forward run to differentiate");
+    *dJ_dtb = a->dJ_dtb;
     return R3D_OK;
 }
 

@@ -39,7 +39,7 @@ SMOOTH = ["--re", "20", "--pr", "1"]  # the advective dt limit binds
 NSTEPS, EVERY = 6, 4
 # finite-difference steps, relative to the field's magnitude or the parameter
 EPS = {"ru": 1e-6, "rv": 1e-6, "rw": 1e-6, "ro": 1e-6, "tt": 1e-6,
-       "bx": 1e-3, "by": 1e-3, "bz": 1e-3, "re": 1e-4, "theta": 1e-4}
+       "bx": 1e-3, "by": 1e-3, "bz": 1e-3, "re": 1e-4, "theta": 1e-4, "tb": 1e-4}
 PRIM = ["ru", "rv", "rw", "ro", "tt"]
 
 
@@ -72,18 +72,19 @@ def fixed_setup():
     return run, x0, dt
 
 
-def fd_check(cfg, met, x0, field, grads, dre, dth, fixed_dt=0.0, tol=1e-6):
+def fd_check(cfg, met, x0, field, grads, dre, dth, fixed_dt=0.0, tol=1e-6, dtb=None):
     """Central difference of J along a direction of one primary, or in Re /
-    theta, against the adjoint's directional derivative.  The direction is
+    theta / the bottom temperature tb, against the adjoint's directional
+    derivative.  The direction is
     the adjoint gradient scaled to unit maximum plus uniform noise in [-1, 1],
     times the field's largest magnitude; the step is EPS[field] of that."""
     eps = EPS[field]
-    if field in ("re", "theta"):
-        v0 = cfg.re if field == "re" else cfg.theta
+    if field in ("re", "theta", "tb"):
+        v0 = getattr(cfg, field)
         h = eps * v0
         fd = (run_J(pkg.cfg_with(cfg, **{field: v0 + h}), met, x0, fixed_dt=fixed_dt) -
               run_J(pkg.cfg_with(cfg, **{field: v0 - h}), met, x0, fixed_dt=fixed_dt)) / (2 * h)
-        ad = dre if field == "re" else dth
+        ad = {"re": dre, "theta": dth, "tb": dtb}[field]
     else:
         rng = np.random.default_rng(sum(map(ord, field)))
         g = grads[field][1:-1, 1:-1, 1:-1]
@@ -110,7 +111,7 @@ def run_J(cfg, metrics, x0, nsteps=NSTEPS, fixed_dt=0.0):
     return J
 
 
-@pytest.mark.parametrize("field", PRIM + ["re", "theta"])
+@pytest.mark.parametrize("field", PRIM + ["re", "theta", "tb"])
 def test_fixed_dt_gradient_matches_finite_differences(fixed_setup, field):
     """With a fixed step J is smooth: the transpose of the fluxes and the
     boundary conditions alone, at the default Re = 2, Pr = 0.05."""
@@ -122,8 +123,9 @@ def test_fixed_dt_gradient_matches_finite_differences(fixed_setup, field):
     adj.set_initial(x0)
     adj.forward(NSTEPS)
     grads, dre, dth = adj.backward()
+    dtb = adj.dJ_dtb()
     adj.close()
-    fd_check(cfg, met, x0, field, grads, dre, dth, fixed_dt=dt)
+    fd_check(cfg, met, x0, field, grads, dre, dth, fixed_dt=dt, dtb=dtb)
 
 
 def test_adjoint_forward_is_the_plain_path(setup):
@@ -340,3 +342,46 @@ def test_fixed_dt_gradient_on_a_ragged_mhd_grid(field):
     grads, dre, dth = adj.backward()
     adj.close()
     fd_check(cfg, met, x0, field, grads, dre, dth, fixed_dt=dt)
+
+
+@pytest.mark.parametrize("mag", [False, True], ids=["hydro", "mhd"])
+def test_total_theta_derivative_through_the_static_profile(mag):
+    """theta sets the static profile (rho, T and the bottom temperature tb) as
+    well as the heat-flux conditions.  J(theta) = J of the profile built at
+    theta with the reference's temperature perturbation on top: the autograd
+    total (adjoint for the run, central difference of the host setup for the
+    profile, dJ/dtb from the transposed boundary pass) against central
+    differences of the whole pipeline.  Fixed dt, so J is smooth in theta."""
+    torch = pytest.importorskip("torch")
+    at = importlib.import_module("rast3dmhd_amd.adjoint_torch")
+    args = MAG_ARGS if mag else ARGS
+    say(f"[gpu] total dJ/dtheta through the static profile ({'mhd' if mag else 'hydro'})")
+    sim = pkg.Run(args).make_sim()
+    sim.step(1)
+    dt = 0.8 * sim.scalars()["dt"]
+    sim.close()
+    prob = at.KineticEnergyRun(args, NSTEPS, EVERY, fixed_dt=dt)
+    th0 = prob.cfg.theta
+    t_static = pkg.Run(args + ["--ampt", "0"]).state()["tt"][1:-1, 1:-1, 1:-1]
+    delta = prob.initial_tt() / torch.from_numpy(t_static.copy()) - 1.0
+    re = torch.tensor(prob.cfg.re, dtype=torch.float64)
+
+    def J_of(theta):
+        st = prob.static_state(theta)
+        fields = {n: st[n] for n in ("ro", "bx", "by", "bz")}
+        fields["tt"] = st["tt"] * (1.0 + delta)
+        return prob.kinetic_energy(fields, re, theta, st["tb"])
+
+    th = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+    J = J_of(th)
+    J.backward()
+    ad = float(th.grad)
+    h = EPS["theta"] * th0
+    with torch.no_grad():
+        fd = float(J_of(torch.tensor(th0 + h, dtype=torch.float64)) -
+                   J_of(torch.tensor(th0 - h, dtype=torch.float64))) / (2 * h)
+    with torch.no_grad():
+        st = prob.static_state(torch.tensor(th0, dtype=torch.float64))
+    say(f"[gpu]   J {float(J.detach()):.12e} autograd {ad:.12e} fd {fd:.12e}")
+    assert abs(ad - fd) <= 1e-6 * abs(fd)
+    assert abs(float(st["tb"]) - prob.cfg.tb) <= 1e-12 * abs(prob.cfg.tb)
