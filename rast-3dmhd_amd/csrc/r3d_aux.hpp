@@ -92,18 +92,18 @@ __device__ __forceinline__ void bound_block(const StageArgs& a, const Consts& C,
             }
 
             if (a.zmode == 0 && a.stage < 2) {
-                // mix for the next stage with the post-BCON state
-                const int nf = C.lmag ? 8 : 5;
-                const long fo = (side ? nf : 0) * g.pp + XOFF + i + g.px * j;
-                a.z.q[QRU][n] = ru + cz * a.fb[fo + 0 * g.pp];
-                a.z.q[QRV][n] = rv + cz * a.fb[fo + 1 * g.pp];
-                a.z.q[QRW][n] = rw + cz * a.fb[fo + 2 * g.pp];
-                a.z.q[QRO][n] = ro + cz * a.fb[fo + 3 * g.pp];
-                a.z.q[QTT][n] = tt + cz * a.fb[fo + 4 * g.pp];
+                // mix for the next stage with the post-BCON state; the stage
+                // kernel left F in Z on this plane
+                double* const* Z = a.z.q;
+                Z[QRU][n] = ru + cz * Z[QRU][n];
+                Z[QRV][n] = rv + cz * Z[QRV][n];
+                Z[QRW][n] = rw + cz * Z[QRW][n];
+                Z[QRO][n] = ro + cz * Z[QRO][n];
+                Z[QTT][n] = tt + cz * Z[QTT][n];
                 if (C.lmag) {
-                    a.z.q[QBX][n] = bx + cz * a.fb[fo + 5 * g.pp];
-                    a.z.q[QBY][n] = by + cz * a.fb[fo + 6 * g.pp];
-                    a.z.q[QBZ][n] = bz + cz * a.fb[fo + 7 * g.pp];
+                    Z[QBX][n] = bx + cz * Z[QBX][n];
+                    Z[QBY][n] = by + cz * Z[QBY][n];
+                    Z[QBZ][n] = bz + cz * Z[QBZ][n];
                 }
             }
             if (a.stage == 2 && a.want_dt) {

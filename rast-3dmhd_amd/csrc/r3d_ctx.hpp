@@ -48,7 +48,6 @@ struct r3d_hip_ctx {
     double* xs[2][r3d::NXF] = {};  // X ping-pong; xs[cur] is the current state
     double* z[r3d::NXF] = {};
     int cur = 0;
-    double* fb = nullptr;
     double* partial = nullptr;
     r3d::DevScal* scal = nullptr;
     double* metric_buf = nullptr;

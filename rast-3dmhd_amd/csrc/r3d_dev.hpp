@@ -77,7 +77,6 @@ struct State {
 struct StageArgs {
     Geo g;
     State xin, xout, z;
-    double* fb;        // F on the two BCON planes: [side][field][j][px]
     double* partial;   // per-block dt partial extrema (stage 3)
     DevScal* scal;
     int stage;         // 0, 1, 2

@@ -26,7 +26,6 @@ struct EnsMember {
     Geo g;
     State x[2];   // X ping-pong
     State z;
-    double* fb;
     double* partial;
     DevScal* scal;
     int ntx, nty, nkc, kc, nblocks, nbb, tile0, npart_dt;
@@ -43,7 +42,6 @@ __device__ __forceinline__ StageArgs member_args(const EnsMember& e, int cur, in
     a.xin = e.x[cur];
     a.xout = e.x[cur ^ 1];
     a.z = e.z;
-    a.fb = e.fb;
     a.partial = e.partial;
     a.scal = e.scal;
     a.stage = stage;
@@ -147,7 +145,6 @@ int build_table(r3d_ens_ctx* e) {
         for (int s = 0; s < 2; ++s)
             for (int f = 0; f < NXF; ++f) d.x[s].q[f] = c->xs[s][f];
         for (int f = 0; f < NXF; ++f) d.z.q[f] = c->z[f];
-        d.fb = c->fb;
         d.partial = e->part[m];
         d.scal = c->scal;
         d.ntx = (c->cfg.npx + 63) / 64;

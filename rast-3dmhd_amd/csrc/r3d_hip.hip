@@ -212,7 +212,6 @@ StageArgs stage_args(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
         a.xout.q[f] = c->xs[c->cur ^ 1][f];
         a.z.q[f] = c->z[f];
     }
-    a.fb = c->fb;
     a.partial = c->partial;
     a.scal = c->scal;
     a.stage = stage;
@@ -419,7 +418,6 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
         if (hipMemsetAsync(c->z[f2], 0, fe * sizeof(double), c->stream) != hipSuccess)
             return fail(set_err(R3D_EHIP, "hipMemset failed"));
     }
-    if ((rc = dev_alloc(c, &c->fb, (size_t)2 * c->nf * c->g.pp + 16))) return fail(rc);
     const size_t npart = std::max<size_t>((size_t)c->nblocks + 2 * c->nbb, DT_BLOCKS);
     if ((rc = dev_alloc(c, &c->partial, npart * NPART))) return fail(rc);
     {
@@ -454,7 +452,6 @@ int r3d_hip_destroy(r3d_hip_ctx* c) {
             if (c->xs[s][f]) (void)hipFree(c->xs[s][f]);
     for (int f = 0; f < NXF; ++f)
         if (c->z[f]) (void)hipFree(c->z[f]);
-    if (c->fb) (void)hipFree(c->fb);
     if (c->partial) (void)hipFree(c->partial);
     if (c->scal) (void)hipFree(c->scal);
     if (c->metric_buf) (void)hipFree(c->metric_buf);

@@ -67,14 +67,20 @@ struct Col {
 // the workgroup (field pointer + k * plane bytes, held in scalar registers)
 // plus one 32-bit per-lane byte offset within the plane, so that no per-lane
 // 64-bit address has to stay live across the z loop.
+//
+// They go through address-space-1 (global) pointers.  A generic pointer the
+// compiler cannot trace to a kernel argument becomes a FLAT access, which
+// also counts on the LDS counter: every later LDS read of the flux blocks
+// would then wait (lgkmcnt(0)) until the preceding HBM store has completed.
+typedef __attribute__((address_space(1))) double gdouble;
 __device__ __forceinline__ const char* plane_base(const double* f, long pp, int k) {
     return reinterpret_cast<const char*>(f) + (size_t)pp * 8u * (size_t)k;
 }
 __device__ __forceinline__ double ldp(const double* f, long pp, int k, uint32_t voff) {
-    return *reinterpret_cast<const double*>(plane_base(f, pp, k) + voff);
+    return *(const gdouble*)(plane_base(f, pp, k) + voff);
 }
 __device__ __forceinline__ void stp(double* f, long pp, int k, uint32_t voff, double v) {
-    *reinterpret_cast<double*>(const_cast<char*>(plane_base(f, pp, k)) + voff) = v;
+    *(gdouble*)(const_cast<char*>(plane_base(f, pp, k)) + voff) = v;
 }
 
 template <bool MAG>
@@ -208,10 +214,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
         a.scal->dt = a.scal->dt_next;
         a.scal->umach = a.scal->umach_next;
     }
-    const int ty = t % a.nty;
-    const int rem = t / a.nty;
-    const int tx = rem % a.ntx;
-    const int kz = rem / a.ntx;
+    // x fastest: the tiles an XCD runs at the same time are x-neighbours
+    // (whole tile rows), whose x-halo columns lie in each other's cache lines
+    const int tx = t % a.ntx;
+    const int rem = t / a.ntx;
+    const int ty = rem % a.nty;
+    const int kz = rem / a.nty;
     const Geo g = a.g;
     const int i0 = 1 + 64 * tx, j0 = 1 + TY * ty;
     const int k0 = 1 + a.kc * kz;
@@ -352,11 +360,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 if (ZMODE == 1) {
                     stp(a.z.q[f], g.pp, k, soff, cur);
                 } else if (ZMODE == 0 && STAGE < 2) {
-                    // on the two BCON planes F is parked for the boundary
-                    // kernel, which mixes after BCON (uniform select)
-                    double* dst = bplane ? a.fb : a.z.q[f];
-                    const int kp = bplane ? (k == 1 ? 0 : NF) + f : k;
-                    stp(dst, g.pp, kp, soff, bplane ? fval : x + cz * fval);
+                    // on the two BCON planes Z' holds F until the boundary
+                    // kernel mixes it with the post-BCON state
+                    stp(a.z.q[f], g.pp, k, soff, bplane ? fval : x + cz * fval);
                 }
             };
 #define LAPX(e, ww, c2) ((e - ww) * hxd * d2xx + (e - c2 + ww) * C.h2x * dxx * dxx)
