@@ -72,6 +72,10 @@ struct Col {
 // compiler cannot trace to a kernel argument becomes a FLAT access, which
 // also counts on the LDS counter: every later LDS read of the flux blocks
 // would then wait (lgkmcnt(0)) until the preceding HBM store has completed.
+//
+// Stores and the Z loads are non-temporal (streaming): X' and Z' are not read
+// again in this launch and Z is read by its own lane only, so they should not
+// displace the X planes that neighbouring tiles still read through L2.
 typedef __attribute__((address_space(1))) double gdouble;
 __device__ __forceinline__ const char* plane_base(const double* f, long pp, int k) {
     return reinterpret_cast<const char*>(f) + (size_t)pp * 8u * (size_t)k;
@@ -79,8 +83,11 @@ __device__ __forceinline__ const char* plane_base(const double* f, long pp, int 
 __device__ __forceinline__ double ldp(const double* f, long pp, int k, uint32_t voff) {
     return *(const gdouble*)(plane_base(f, pp, k) + voff);
 }
+__device__ __forceinline__ double ldp_stream(const double* f, long pp, int k, uint32_t voff) {
+    return __builtin_nontemporal_load((const gdouble*)(plane_base(f, pp, k) + voff));
+}
 __device__ __forceinline__ void stp(double* f, long pp, int k, uint32_t voff, double v) {
-    *(gdouble*)(const_cast<char*>(plane_base(f, pp, k)) + voff) = v;
+    __builtin_nontemporal_store(v, (gdouble*)(const_cast<char*>(plane_base(f, pp, k)) + voff));
 }
 
 template <bool MAG>
@@ -289,7 +296,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
         const Raw rnn = load_raw<MAG>(a.xin, g.pp, min(k + 2, g.npz + 1), voff);
         double zc[NF];
         if (STAGE > 0 && heavy)
-            for (int f = 0; f < NF; ++f) zc[f] = ldp(a.z.q[f], g.pp, k, voff);
+            for (int f = 0; f < NF; ++f) zc[f] = ldp_stream(a.z.q[f], g.pp, k, voff);
         const Col cp = derive(rn);
         const int kl = k - k0;
         const double dzz = zm[0][kl], d2zz = zm[1][kl];
