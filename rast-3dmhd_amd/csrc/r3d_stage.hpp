@@ -213,6 +213,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // global-memory round trip on every plane.
     constexpr int KCMAX = 128;
     __shared__ double zm[5][KCMAX];  // dzz, d2zz, ddz, rkapa, dkapa
+    // Stage 3: min(ddx, ddy) of this lane's column, kept in LDS rather than in
+    // a register across the z loop.  Held in a register it is spilled, and
+    // the scratch reload after a plane's stores waits for all of them (vector
+    // memory operations complete in issue order); an LDS read does not.
+    __shared__ double ddxy_s[STAGE == 2 ? ROWS : 1][66];
 
     // Stage 1 takes the dt reduced at the end of the previous step (or by the
     // standalone dt pass) and makes it the step's dt.
@@ -277,7 +282,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // x/y metric factors of this column (src/fluxes.cpp:90, 250-254)
     const double dxx = M.dxx[li], d2xx = M.d2xx[li];
     const double dyy = M.dyy[lj], d2yy = M.d2yy[lj];
-    const double ddxy = (STAGE == 2) ? dmin(M.ddx[li], M.ddy[lj]) : 0.0;
+    if (STAGE == 2 && heavy) ddxy_s[r][c] = dmin(M.ddx[li], M.ddy[lj]);
 
     Col cm = derive(load_raw<MAG>(a.xin, g.pp, k0 - 1, voff));
     Col cc = derive(load_raw<MAG>(a.xin, g.pp, k0, voff));
@@ -603,6 +608,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
             if (STAGE == 2 && a.want_dt) {
                 // evaluated on every lane, folded in only on output cells
                 double q[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
+                asm volatile("" ::: "memory");  // keeps the read inside the loop
+                const double ddxy = ddxy_s[r][c];
                 const double dd = dmin(ddxy, zm[2][kl]);
                 double bxv = 0.0, byv = 0.0, bzv = 0.0;
                 if constexpr (MAG) {
