@@ -56,6 +56,11 @@ struct Raw {
     double ru, rv, rw, ro, tt, bx, by, bz;
 };
 
+template <bool B>
+struct BoolConst {
+    static constexpr bool value = B;
+};
+
 // Per-plane values of one column (reference src/fluxes.cpp:131-166: P = RO*TT
 // before the inversion, 1/RO, velocities = momentum * (1/RO)); ruw..rww are
 // the vertical advective fluxes m*w of src/fluxes.cpp:183-185.
@@ -213,11 +218,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // global-memory round trip on every plane.
     constexpr int KCMAX = 128;
     __shared__ double zm[5][KCMAX];  // dzz, d2zz, ddz, rkapa, dkapa
-    // Stage 3: min(ddx, ddy) of this lane's column, kept in LDS rather than in
-    // a register across the z loop.  Held in a register it is spilled, and
-    // the scratch reload after a plane's stores waits for all of them (vector
-    // memory operations complete in issue order); an LDS read does not.
+    // Stage 3: the column's min(ddx, ddy) and the lane's running dt extrema
+    // live in LDS rather than in registers across the z loop.  Held in
+    // registers they are spilled, and a scratch reload after a plane's stores
+    // waits for all of them (vector memory operations complete in issue
+    // order); an LDS access does not.
     __shared__ double ddxy_s[STAGE == 2 ? ROWS : 1][66];
+    constexpr int NPM = MAG ? NPART : NPART - 1;  // extrema in use (no resistive limit without B)
+    __shared__ double pm_s[STAGE == 2 ? NPM : 1][STAGE == 2 ? TY : 1][64];
 
     // Stage 1 takes the dt reduced at the end of the previous step (or by the
     // standalone dt pass) and makes it the step's dt.
@@ -237,7 +245,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     const int k0 = 1 + a.kc * kz;
     const int k1 = min(g.npz, k0 + a.kc - 1);
 
-    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    // the wave index is uniform over the wave; from a scalar register, the
+    // role branch below is a scalar branch
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
     int r, c, gi, gj;
     bool act;
     if (wv < ROWS) {
@@ -282,7 +292,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // x/y metric factors of this column (src/fluxes.cpp:90, 250-254)
     const double dxx = M.dxx[li], d2xx = M.d2xx[li];
     const double dyy = M.dyy[lj], d2yy = M.d2yy[lj];
-    if (STAGE == 2 && heavy) ddxy_s[r][c] = dmin(M.ddx[li], M.ddy[lj]);
+    if (STAGE == 2 && heavy) {
+        ddxy_s[r][c] = dmin(M.ddx[li], M.ddy[lj]);
+        pm_s[0][r - 1][ln] = 0.0;
+        for (int q = 1; q < NPM; ++q) pm_s[q][r - 1][ln] = 1e300;
+    }
 
     Col cm = derive(load_raw<MAG>(a.xin, g.pp, k0 - 1, voff));
     Col cc = derive(load_raw<MAG>(a.xin, g.pp, k0, voff));
@@ -291,6 +305,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
 
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
 
+    // The z loop is instantiated twice: for the flux rows (HEAVY) and for the
+    // halo waves, which only load and publish.  Both run one barrier per
+    // plane over the same planes.  With one loop for both, the compiler's
+    // wait counts at the loop head would be those of the halo path, which
+    // issues no stores: a flux wave would then wait there for all of the
+    // previous plane's stores to complete.
+    const int r_all = r, c_all = c;
+    auto zloop = [&](auto heavy_c) {
+    constexpr bool HEAVY = decltype(heavy_c)::value;
+    // a flux row's LDS row is its (uniform) wave index and its column the
+    // lane + 1: no per-lane row index has to stay live in that loop
+    const int r = HEAVY ? wv : r_all;
+    const int c = HEAVY ? ln + 1 : c_all;
     for (int k = k0; k <= k1; ++k) {
         const int buf = k & 1;
         // Keep the lane offsets opaque to loop-invariant code motion: every
@@ -300,7 +327,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
         asm volatile("" : "+v"(soff));
         const Raw rnn = load_raw<MAG>(a.xin, g.pp, min(k + 2, g.npz + 1), voff);
         double zc[NF];
-        if (STAGE > 0 && heavy)
+        if constexpr (STAGE > 0 && HEAVY)
             for (int f = 0; f < NF; ++f) zc[f] = ldp_stream(a.z.q[f], g.pp, k, voff);
         const Col cp = derive(rn);
         const int kl = k - k0;
@@ -310,7 +337,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
         const double wz = (cp.w - cm.w) * C.hz * dzz;
         const double uz = (cp.u - cm.u) * C.hz * dzz;
         const double vz = (cp.v - cm.v) * C.hz * dzz;
-        if (act) {
+        if (HEAVY || act) {
             sh[buf][SH_RU][r][c] = cc.ru;
             sh[buf][SH_RV][r][c] = cc.rv;
             sh[buf][SH_RW][r][c] = cc.rw;
@@ -330,7 +357,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
         }
         __syncthreads();
 
-        if (heavy) {
+        if constexpr (HEAVY) {
 #define SH(q, rr, cc_) sh[buf][q][rr][cc_]
             // One flux component at a time: each block reads only its own LDS
             // operands, accumulates its terms in the reference's order, applies
@@ -608,9 +635,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
             if (STAGE == 2 && a.want_dt) {
                 // evaluated on every lane, folded in only on output cells
                 double q[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
-                asm volatile("" ::: "memory");  // keeps the read inside the loop
-                const double ddxy = ddxy_s[r][c];
-                const double dd = dmin(ddxy, zm[2][kl]);
+                asm volatile("" ::: "memory");  // keeps the LDS values in LDS
+                const double dd = dmin(ddxy_s[r][c], zm[2][kl]);
                 double bxv = 0.0, byv = 0.0, bzv = 0.0;
                 if constexpr (MAG) {
                     bxv = xv[QBX];
@@ -620,8 +646,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 dt_terms<MAG>(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], bxv, byv, bzv, dd,
                               zm[3][kl], q);
                 const bool use = outp && !bplane;
-                pm[0] = use ? dmax(pm[0], q[0]) : pm[0];
-                for (int v = 1; v < NPART; ++v) pm[v] = use ? dmin(pm[v], q[v]) : pm[v];
+                const double p0 = pm_s[0][r - 1][ln];
+                pm_s[0][r - 1][ln] = use ? dmax(p0, q[0]) : p0;
+                for (int v = 1; v < NPM; ++v) {
+                    const double pv = pm_s[v][r - 1][ln];
+                    pm_s[v][r - 1][ln] = use ? dmin(pv, q[v]) : pv;
+                }
             }
         }
 
@@ -630,8 +660,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
         cc = cp;
         rn = rnn;
     }
+    };
+    if (heavy)
+        zloop(BoolConst<true>{});
+    else
+        zloop(BoolConst<false>{});
 
-    if (STAGE == 2 && a.want_dt) block_partials<NW>(pm, red, a.partial, t);
+    if (STAGE == 2 && a.want_dt) {
+        if (heavy)
+            for (int q = 0; q < NPM; ++q) pm[q] = pm_s[q][r - 1][ln];
+        block_partials<NW>(pm, red, a.partial, t);
+    }
 }
 
 // One RK stage of one run: block -> tile with the XCD-aware order.
