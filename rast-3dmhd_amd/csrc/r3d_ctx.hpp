@@ -14,12 +14,13 @@
 
 namespace r3d {
 
-constexpr int TILE_Y = 9;  // rows per hydro stage tile (waves 1..9 compute fluxes)
-// Hydro tile rows for a grid npy cells high: 9 (12 waves, every wave slot of a
-// CU at 3 per SIMD) unless 8 leaves fewer rows of the last tile idle.
+constexpr int TILE_Y = 9;  // the larger hydro stage tile (waves 1..9 compute fluxes)
+// Hydro tile rows for a grid npy cells high: 8 (11 waves; the 8 flux waves
+// sit two per SIMD, which measures faster than 9 rows with three flux waves on
+// one SIMD) unless 9 leaves fewer rows of the last tile idle.
 inline int hydro_tile_rows(int npy) {
     const int w9 = (npy + 8) / 9 * 9 - npy, w8 = (npy + 7) / 8 * 8 - npy;
-    return w8 < w9 ? 8 : TILE_Y;
+    return w9 < w8 ? TILE_Y : 8;
 }
 // MHD tiles: 8 waves per workgroup (two per SIMD), so that the larger MHD
 // working set gets 256 VGPRs per lane instead of 168
