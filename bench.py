@@ -185,23 +185,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--npx", type=int, default=504)
-    ap.add_argument("--npy", type=int, default=504)
-    ap.add_argument("--npz", type=int, default=2048)
-    ap.add_argument("--npey", type=int, default=4)
-    ap.add_argument("--npez", type=int, default=2)
+    # grid and virtual ranks; unset sizes take the mode's workload: hydro
+    # 504x504x2048 (4x2), --mhd 252x252x512 (2x4), --adjoint 128x128x256 (2x4)
+    ap.add_argument("--npx", type=int, default=None)
+    ap.add_argument("--npy", type=int, default=None)
+    ap.add_argument("--npz", type=int, default=None)
+    ap.add_argument("--npey", type=int, default=None)
+    ap.add_argument("--npez", type=int, default=None)
     ap.add_argument("--mhd", action="store_true",
                     help="rotating MHD with the magnetic layer (Rm=50, beta=10, ampb=1, bfh=2, "
-                         "bzp=10, ry=0.1); use with --npx 252 --npy 252 --npz 512 --npey 2 "
-                         "--npez 4 for the MHD configuration")
+                         "bzp=10, ry=0.1); default size 252x252x512 (2x4 virtual ranks)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="M",
                     help="batched ensemble of M seeded hydro members (sizes 48..96 x 96..192, "
                          "Re in [1,4], theta in [0.1,0.3]) against the same members run one "
                          "after another through the plain path")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the discrete adjoint (J = final kinetic energy, checkpoints every "
-                         "20 steps) against its forward run; use with --npx 128 --npy 128 "
-                         "--npz 256 --npey 2 --npez 4 --steps 200")
+                         "20 steps) against its forward run; default size 128x128x256 (2x4 "
+                         "virtual ranks); the specified workload adds --mhd --steps 200")
     ap.add_argument("--variant", type=int, default=0, choices=(0, 1),
                     help="0: bit-exact stage kernel (default); 1: FMA-contracted stage kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -209,6 +210,15 @@ def main():
                     help="before the run, time one device-to-device copy of this many GiB "
                          "(a plain copy of known size, used to scale profiler byte counters)")
     a = ap.parse_args()
+    if a.adjoint:
+        sizes = (128, 128, 256, 2, 4)
+    elif a.mhd:
+        sizes = (252, 252, 512, 2, 4)
+    else:
+        sizes = (504, 504, 2048, 4, 2)
+    for name, v in zip(("npx", "npy", "npz", "npey", "npez"), sizes):
+        if getattr(a, name) is None:
+            setattr(a, name, v)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
               "the multi-GPU sharding is a design, not a launched path", file=sys.stderr)
