@@ -105,6 +105,23 @@ int r3d_hip_upload_global(r3d_hip_ctx* ctx, const double* const* fields);
 int r3d_hip_download_rank(r3d_hip_ctx* ctx, int rank, double* const* fields);
 int r3d_hip_download_global(r3d_hip_ctx* ctx, double* const* fields);
 
+/* Asynchronous snapshot of the current state, for checkpoint dumps that
+ * overlap stepping (the reference's dumps stop the run for a blocking
+ * download, src/main.cpp:258-264).  begin copies X
+ * and Z (dense global fields) and the scalars on the device, ordered after
+ * every step issued so far, then to pinned host memory on a second stream;
+ * it returns at once, and steps issued afterwards run while the copy is in
+ * flight.  snapshot_rank waits for that copy and returns one virtual rank as
+ * r3d_hip_download_rank would have at the snapshot's point (fields[R3D_RU..
+ * R3D_ZBZ], NULL entries skipped) plus its scalars (s may be NULL).  The
+ * buffers (2 x nfields global fields on the device and pinned on the host)
+ * are allocated on the first begin.  A later begin reuses them: finish the
+ * snapshot_rank calls of one snapshot before the next begin.  snapshot_rank
+ * may be called from another host thread than the one stepping. */
+int r3d_hip_snapshot_begin(r3d_hip_ctx* ctx);
+int r3d_hip_snapshot_rank(r3d_hip_ctx* ctx, int rank, double* const* fields,
+                          r3d_hip_scalars* s);
+
 int r3d_hip_set_scalars(r3d_hip_ctx* ctx, const r3d_hip_scalars* s);
 int r3d_hip_get_scalars(r3d_hip_ctx* ctx, r3d_hip_scalars* s);
 

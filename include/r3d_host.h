@@ -24,7 +24,9 @@ extern "C" {
 
 typedef struct r3d_host r3d_host;
 
-/* Parse flags, derive constants, build metrics and the initial state. */
+/* Parse flags, derive constants, build metrics and the initial state (the
+ * static initial condition, or with --nstart N > 0 the state of the dumps
+ * --dumpdir/gstate.step.<rank>.<N>, --dumpdir defaulting to "."). */
 r3d_host* r3d_host_create(int argc, const char* const* argv);
 void r3d_host_free(r3d_host* h);
 const char* r3d_host_last_error(void);
@@ -50,6 +52,21 @@ int r3d_host_rank_metrics(const r3d_host* h, int rank, double* const* out);
  * from downloaded fields[R3D_RU..R3D_ZBZ] (NULL = zeros) and the scalars. */
 int r3d_host_write_gstate(const r3d_host* h, int rank, const char* path,
                           const double* const* fields, const r3d_hip_scalars* s);
+
+/* Read one virtual rank's gstate dump (the same format; a dump of the
+ * reference CPU build or of r3d_host_write_gstate) into fields[R3D_RU..
+ * R3D_ZBZ] (NULL entries skipped, each nx*ny*nz doubles) and s.  Fails
+ * unless the header describes this rank of this run (local extents, rank
+ * coordinates, LMAG, boundary switches) and every array has its length.
+ * The reference has no reader: its restart stops with "restart (NSTART!=0)
+ * is not yet ported" (src/main.cpp:148-153). */
+int r3d_host_read_gstate(const r3d_host* h, int rank, const char* path,
+                         double* const* fields, r3d_hip_scalars* s);
+
+/* Starting scalars of the run: zero for a fresh start; with --nstart N
+ * (restart from --dumpdir/gstate.step.<rank>.<N>, every rank's dump) those
+ * of the dumps, which also supply the global state. */
+int r3d_host_scalars(const r3d_host* h, r3d_hip_scalars* out);
 
 #ifdef __cplusplus
 }

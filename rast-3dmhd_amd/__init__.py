@@ -81,6 +81,8 @@ def load():
         "r3d_hip_download_global": [vp, ctypes.POINTER(dp)],
         "r3d_hip_set_scalars": [vp, ctypes.POINTER(Scalars)],
         "r3d_hip_get_scalars": [vp, ctypes.POINTER(Scalars)],
+        "r3d_hip_snapshot_begin": [vp],
+        "r3d_hip_snapshot_rank": [vp, ip, ctypes.POINTER(dp), ctypes.POINTER(Scalars)],
         "r3d_hip_step": [vp, ip, ip],
         "r3d_hip_sync": [vp],
         "r3d_hip_set_timing": [vp, ip],
@@ -219,6 +221,22 @@ class HipSim:
 
     def step(self, nsteps=1, nstages=3):
         _check(load().r3d_hip_step(self.h, nsteps, nstages))
+
+    def snapshot_begin(self):
+        """Start an asynchronous snapshot of the current state (include/r3d_hip.h)."""
+        _check(load().r3d_hip_snapshot_begin(self.h))
+
+    def snapshot_rank(self, rank, names=None):
+        """Wait for the snapshot; one virtual rank's fields and the scalars."""
+        names = names or FIELD_NAMES
+        out = {n: np.empty((self.nz, self.ny, self.nx)) for n in names}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 16)()
+        for n in names:
+            ptrs[FIELD_NAMES.index(n)] = _dptr(out[n])
+        s = Scalars()
+        _check(load().r3d_hip_snapshot_rank(self.h, rank, ptrs, ctypes.byref(s)))
+        return out, {"dt": s.dt, "timc": s.timc, "timt": s.timt, "timi": s.timi,
+                     "umach": s.umach, "nit": s.nit}
 
     def sync(self):
         _check(load().r3d_hip_sync(self.h))
@@ -478,6 +496,10 @@ def load_host():
     L.r3d_host_write_gstate.argtypes = [vp, ctypes.c_int, ctypes.c_char_p,
                                         ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
                                         ctypes.POINTER(Scalars)]
+    L.r3d_host_read_gstate.argtypes = [vp, ctypes.c_int, ctypes.c_char_p,
+                                       ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
+                                       ctypes.POINTER(Scalars)]
+    L.r3d_host_scalars.argtypes = [vp, ctypes.POINTER(Scalars)]
     _host = L
     return L
 
@@ -565,12 +587,59 @@ class Run:
                                              ctypes.byref(s)) != 0:
             raise HipError(-1, load_host().r3d_host_last_error().decode())
 
+    def scalars(self):
+        """Starting scalars: zero, or those of the dumps on a restart (--nstart)."""
+        s = Scalars()
+        if load_host().r3d_host_scalars(self.h, ctypes.byref(s)) != 0:
+            raise HipError(-1, load_host().r3d_host_last_error().decode())
+        return {"dt": s.dt, "timc": s.timc, "timt": s.timt, "timi": s.timi,
+                "umach": s.umach, "nit": s.nit}
+
+    def read_gstate(self, rank, path):
+        """One virtual rank's gstate dump: (fields by FIELD_NAMES, scalars)."""
+        nry, nrz = self.npy // self.npey, self.npz // self.npez
+        shape = (nrz + 2, nry + 2, self.npx + 2)
+        out = {n: np.zeros(shape) for n in FIELD_NAMES}
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 16)(*[_dptr(out[n]) for n in FIELD_NAMES])
+        s = Scalars()
+        if load_host().r3d_host_read_gstate(self.h, rank, path.encode(), ptrs,
+                                            ctypes.byref(s)) != 0:
+            raise HipError(-1, load_host().r3d_host_last_error().decode())
+        return out, {"dt": s.dt, "timc": s.timc, "timt": s.timt, "timi": s.timi,
+                     "umach": s.umach, "nit": s.nit}
+
+    def dump_async(self, sim, directory, tag="step"):
+        """Checkpoint without stopping the run: snapshot the device state now
+        and write every virtual rank's gstate dump
+        (directory/gstate.<tag>.<rank>.<nit>, the reference's names) on a
+        background thread while the caller keeps stepping.  Returns the
+        started thread; join it before the next dump_async on this sim.  A run
+        started with --nstart <nit> --dumpdir <directory> continues from it."""
+        import threading
+        sim.snapshot_begin()
+        errors = []
+
+        def write():
+            try:
+                for r in range(self.npey * self.npez):
+                    fields, sc = sim.snapshot_rank(r)
+                    self.write_gstate(r, os.path.join(directory, f"gstate.{tag}.{r:03d}.{sc['nit']:04d}"),
+                                      fields, sc)
+            except Exception as e:  # surfaced by join()
+                errors.append(e)
+
+        t = threading.Thread(target=write, daemon=True)
+        t.errors = errors
+        t.start()
+        return t
+
     def make_sim(self, variant=0):
-        """Create the device context and upload this run's metrics and state."""
+        """Create the device context and upload this run's metrics, state and
+        starting scalars."""
         cfg = self.cfg()
         cfg.variant = variant
         sim = HipSim(cfg)
         sim.set_metrics(self.metrics())
         sim.upload_global(self.state(copy=False))
-        sim.set_scalars()
+        sim.set_scalars(**self.scalars())
         return sim

@@ -54,6 +54,15 @@ struct r3d_hip_ctx {
     double* metric_buf = nullptr;
     double* stage_buf = nullptr;
     double* gstage = nullptr;  // one global field, (npx+2)(npy+2)(npz+2), allocated on first use
+    // asynchronous snapshot (r3d_hip_snapshot_begin): X and Z as dense global
+    // fields on the device and in pinned host memory, the scalars, a copy
+    // stream and its events; allocated on first use
+    double* snap_dev = nullptr;
+    double* snap_host = nullptr;
+    r3d::DevScal* snap_scal = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+    bool snap_taken = false;
     r3d::Metrics M{};
     r3d::Consts C{};
     bool metrics_set = false;

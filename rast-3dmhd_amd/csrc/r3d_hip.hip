@@ -457,6 +457,15 @@ int r3d_hip_destroy(r3d_hip_ctx* c) {
     if (c->metric_buf) (void)hipFree(c->metric_buf);
     if (c->stage_buf) (void)hipFree(c->stage_buf);
     if (c->gstage) (void)hipFree(c->gstage);
+    if (c->copy_stream) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
+    if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
+    if (c->snap_done) (void)hipEventDestroy(c->snap_done);
+    if (c->snap_dev) (void)hipFree(c->snap_dev);
+    if (c->snap_host) (void)hipHostFree(c->snap_host);
+    if (c->snap_scal) (void)hipHostFree(c->snap_scal);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return R3D_OK;
@@ -595,6 +604,88 @@ int r3d_hip_download_global(r3d_hip_ctx* c, double* const* fields) {
         }
         double* src = isz ? c->z[fi] : c->xs[c->cur][fi];
         if ((rc = global_copy(c, src, fields[q], true))) return rc;
+    }
+    return R3D_OK;
+}
+
+int r3d_hip_snapshot_begin(r3d_hip_ctx* c) {
+    if (!c) return set_err(R3D_EINVAL, "null context");
+    int rc = check_ready(c);
+    if (rc) return rc;
+    const r3d_hip_cfg& f = c->cfg;
+    const size_t n = (size_t)(f.npx + 2) * (f.npy + 2) * (f.npz + 2);
+    const size_t nsf = 2 * (size_t)c->nf;  // X and Z
+    if (!c->snap_dev) {
+        if ((rc = dev_alloc(c, &c->snap_dev, nsf * n))) return rc;
+        if (hipHostMalloc((void**)&c->snap_host, nsf * n * sizeof(double)) != hipSuccess ||
+            hipHostMalloc((void**)&c->snap_scal, sizeof(DevScal)) != hipSuccess)
+            return set_err(R3D_ENOMEM, "snapshot: pinned host allocation of %zu bytes failed",
+                           nsf * n * sizeof(double));
+        HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+    }
+    // the previous snapshot's copy must be done before its buffer is reused
+    if (c->snap_taken) HIPCHK(hipStreamWaitEvent(c->stream, c->snap_done, 0));
+    if ((rc = fill_ghosts(c))) return rc;
+    const dim3 grid((unsigned)std::min<size_t>(4096, (n + 255) / 256)), blk(256);
+    for (int q = 0; q < c->nf; ++q) {
+        hipLaunchKernelGGL(k_global_copy, grid, blk, 0, c->stream, c->xs[c->cur][q],
+                           c->snap_dev + q * n, c->g, 1);
+        hipLaunchKernelGGL(k_global_copy, grid, blk, 0, c->stream, c->z[q],
+                           c->snap_dev + (c->nf + q) * n, c->g, 1);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->snap_scal, c->scal, sizeof(DevScal), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipEventRecord(c->snap_ready, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->copy_stream, c->snap_ready, 0));
+    HIPCHK(hipMemcpyAsync(c->snap_host, c->snap_dev, nsf * n * sizeof(double),
+                          hipMemcpyDeviceToHost, c->copy_stream));
+    HIPCHK(hipEventRecord(c->snap_done, c->copy_stream));
+    c->snap_taken = true;
+    return R3D_OK;
+}
+
+int r3d_hip_snapshot_rank(r3d_hip_ctx* c, int rank, double* const* fields, r3d_hip_scalars* s) {
+    if (!c || !fields) return set_err(R3D_EINVAL, "null argument");
+    const r3d_hip_cfg& f = c->cfg;
+    if (rank < 0 || rank >= f.npey * f.npez) return set_err(R3D_EINVAL, "rank %d out of range", rank);
+    if (!c->snap_taken) return set_err(R3D_ESTATE, "no snapshot has been taken");
+    HIPCHK(hipEventSynchronize(c->snap_done));
+    const size_t W = (size_t)(f.npx + 2), H = (size_t)(f.npy + 2);
+    const size_t n = W * H * (f.npz + 2);
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const int gj0 = (rank % f.npey) * c->nry, gk0 = (rank / f.npey) * c->nrz;
+    for (int q = 0; q < R3D_NFIELDS; ++q) {
+        if (!fields[q]) continue;
+        const bool isz = q >= R3D_ZRU;
+        const int fi = isz ? q - R3D_ZRU : q;
+        if (fi >= c->nf) {
+            std::memset(fields[q], 0, (size_t)nx * ny * nz * sizeof(double));
+            continue;
+        }
+        // the rank's local box of the global field, as r3d_hip_download_rank
+        // returns it: primaries with every ghost cell, accumulators interior only
+        const double* g = c->snap_host + (size_t)(isz ? c->nf + fi : fi) * n;
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) {
+                double* dst = fields[q] + ((size_t)k * ny + j) * nx;
+                if (isz && (k == 0 || k == nz - 1 || j == 0 || j == ny - 1)) {
+                    std::memset(dst, 0, nx * sizeof(double));
+                    continue;
+                }
+                std::memcpy(dst, g + ((size_t)(gk0 + k) * H + (gj0 + j)) * W, nx * sizeof(double));
+                if (isz) dst[0] = dst[nx - 1] = 0.0;
+            }
+    }
+    if (s) {
+        s->dt = c->snap_scal->dt;
+        s->timc = c->snap_scal->timc;
+        s->timt = c->snap_scal->timt;
+        s->timi = c->snap_scal->timi;
+        s->umach = c->snap_scal->umach;
+        s->nit = c->snap_scal->nit;
     }
     return R3D_OK;
 }

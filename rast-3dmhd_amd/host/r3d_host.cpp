@@ -55,6 +55,8 @@ struct Params {
     double za = 0.65, zb = 1.0, zc = 5.0, zd = 0.2;
     double xcent = 0.0, zcent = 0.0, rmax = 0.0, cmt = 0.0, a_tube = 0.0;
     double lambda = 0.0, vz0 = 0.0, sff = single(0.315f);
+    // restart (--nstart N): directory of the gstate.step.<rank>.<N> dumps
+    std::string dumpdir = ".";
 };
 
 struct Flag {
@@ -112,6 +114,12 @@ int parse(int argc, const char* const* argv, Params& p) {
             }
             return argv[++a];
         };
+        if (key == "--dumpdir") {
+            const char* v = value("--dumpdir");
+            if (!v) return -1;
+            p.dumpdir = v;
+            continue;
+        }
         if (key == "--backend") {
             const char* v = value("--backend");
             if (!v) return -1;
@@ -408,6 +416,7 @@ struct r3d_host {
     double tb = 1.0;
     std::vector<double> m1[14];  // global metrics, gstate order
     std::vector<double> f[8];    // global initial primaries
+    r3d_hip_scalars sc{};        // starting scalars (from the dumps on a restart)
 };
 
 namespace {
@@ -522,13 +531,133 @@ int build_state(r3d_host* h) {
     return 0;
 }
 
+// gstate field order (src/main.cpp:44-93) -> R3D field id (-1: scratch array,
+// not part of the state)
+constexpr int kGstateMap[27] = {R3D_RU, R3D_RV, R3D_RW, R3D_RO, R3D_TT, -1, -1, -1, -1, -1,
+                                -1, -1, -1, R3D_ZRU, R3D_ZRV, R3D_ZRW, R3D_ZRO, R3D_ZTT, -1,
+                                -1, -1, R3D_BX, R3D_BY, R3D_BZ, R3D_ZBX, R3D_ZBY, R3D_ZBZ};
+constexpr int32_t kGstateMagic = 1129928788;
+
+// Read one virtual rank's gstate dump (the format src/main.cpp:44-93 writes)
+// into fields[R3D_RU..R3D_ZBZ] (NULL entries skipped; each nx*ny*nz) and s.
+// The header must describe this rank of this run: magic, 27 + 14 arrays, the
+// local extents, the rank's coordinates, LMAG and the boundary switches.
+int read_gstate(const r3d_host* h, int rank, const char* path, double* const* fields,
+                r3d_hip_scalars* s) {
+    const Params& p = h->p;
+    if (rank < 0 || rank >= p.npey * p.npez) return fail("rank %d out of range", rank);
+    const int nry = p.npy / p.npey, nrz = p.npz / p.npez;
+    const int nx = p.npx + 2, ny = nry + 2, nz = nrz + 2;
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail("cannot open dump '%s'", path);
+    struct Closer {
+        FILE* f;
+        ~Closer() { fclose(f); }
+    } closer{f};
+    fseek(f, 0, SEEK_END);
+    const long fsize = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    auto rd = [&](void* v, size_t bytes) { return fread(v, 1, bytes, f) == bytes; };
+    // skip bytes; false if that passes the end of the file
+    auto skip = [&](int64_t bytes) {
+        return fseek(f, (long)bytes, SEEK_CUR) == 0 && ftell(f) <= fsize;
+    };
+    int32_t head[3], hdr[13];
+    double sc[27];
+    if (!rd(head, sizeof head) || head[0] != kGstateMagic || head[1] != 27 || head[2] != 14)
+        return fail("'%s' is not a gstate dump", path);
+    if (!rd(hdr, sizeof hdr) || !rd(sc, sizeof sc)) return fail("'%s': truncated header", path);
+    const int32_t want[13] = {nx, ny, nz, rank, rank % p.npey, rank / p.npey, hdr[6], p.lmag,
+                              p.ixcon, p.iycon, p.izcon, p.itcon, p.ibcon};
+    static const char* names[13] = {"nx", "ny", "nz", "rank", "mypey", "mypez", "nit", "lmag",
+                                    "ixcon", "iycon", "izcon", "itcon", "ibcon"};
+    for (int q = 0; q < 13; ++q)
+        if (hdr[q] != want[q])
+            return fail("'%s': %s = %d, this run has %d", path, names[q], hdr[q], want[q]);
+    const int64_t cnt = (int64_t)nx * ny * nz;
+    for (int q = 0; q < 27; ++q) {
+        int64_t c;
+        if (!rd(&c, 8)) return fail("'%s': truncated before field %d", path, q);
+        if (c != cnt) return fail("'%s': field %d has a wrong length", path, q);
+        double* dst = kGstateMap[q] >= 0 ? fields[kGstateMap[q]] : nullptr;
+        if (dst ? !rd(dst, (size_t)cnt * 8) : !skip(cnt * 8))
+            return fail("'%s': truncated field %d", path, q);
+    }
+    for (int q = 0; q < 14; ++q) {
+        int64_t c;
+        const int64_t len = q < 4 ? nx : q < 8 ? ny : nz;
+        if (!rd(&c, 8)) return fail("'%s': truncated before metric %d", path, q);
+        if (c != len) return fail("'%s': metric %d has a wrong length", path, q);
+        if (!skip(c * 8)) return fail("'%s': truncated metric %d", path, q);
+    }
+    int64_t end;
+    if (!rd(&end, 8) || end != -1) return fail("'%s': missing end marker", path);
+    s->dt = sc[0];
+    s->timc = sc[1];
+    s->timt = sc[2];
+    s->timi = sc[3];
+    s->umach = sc[4];
+    s->nit = hdr[6];
+    return 0;
+}
+
+// Restart (--nstart N): the global primaries and the scalars from the dumps
+// gstate.step.<rank>.<N> in --dumpdir, one per virtual rank.  A rank supplies
+// its interior and the ghost cells on the global boundary (the periodic
+// images and the outer z planes, whose RO is as the dumped step left it).
+// The static profile still gives tb, the fixed bottom temperature.
+int restart_state(r3d_host* h) {
+    const Params& p = h->p;
+    const int X = p.npx + 2, Y = p.npy + 2, Z = p.npz + 2;
+    const size_t plane = (size_t)X * Y, n = plane * Z;
+    const int nf = p.lmag ? 8 : 5;
+    for (int q = 0; q < nf; ++q) h->f[q].assign(n, 0.0);
+    std::vector<double> T, R;
+    Profile prof(p);
+    if (prof.build(p.npz, T, R)) return -1;
+    h->tb = T[p.npz - 1];
+    const int nry = p.npy / p.npey, nrz = p.npz / p.npez;
+    const int nx = X, ny = nry + 2, nz = nrz + 2;
+    const size_t cnt = (size_t)nx * ny * nz;
+    std::vector<double> loc[R3D_NFIELDS];
+    double* ptr[R3D_NFIELDS] = {};
+    for (int q = 0; q < nf; ++q) {
+        loc[q].assign(cnt, 0.0);
+        ptr[q] = loc[q].data();
+    }
+    for (int r = 0; r < p.npey * p.npez; ++r) {
+        char path[4096];
+        snprintf(path, sizeof path, "%s/gstate.step.%03d.%04d", p.dumpdir.c_str(), r, p.nstart);
+        r3d_hip_scalars s{};
+        if (read_gstate(h, r, path, ptr, &s)) return -1;
+        if (s.nit != p.nstart) return fail("'%s' holds step %d, not %d", path, s.nit, p.nstart);
+        if (r == 0)
+            h->sc = s;
+        else if (memcmp(&s, &h->sc, sizeof s))
+            return fail("'%s': scalars differ from rank 0's dump", path);
+        const int gj0 = (r % p.npey) * nry, gk0 = (r / p.npey) * nrz;
+        for (int k = 0; k < nz; ++k) {
+            const int gk = gk0 + k;
+            if (!(k >= 1 && k <= nz - 2) && gk != 0 && gk != p.npz + 1) continue;
+            for (int j = 0; j < ny; ++j) {
+                const int gj = gj0 + j;
+                if (!(j >= 1 && j <= ny - 2) && gj != 0 && gj != p.npy + 1) continue;
+                for (int q = 0; q < nf; ++q)
+                    memcpy(&h->f[q][(size_t)gk * plane + (size_t)gj * X],
+                           &loc[q][((size_t)k * ny + j) * nx], sizeof(double) * nx);
+            }
+        }
+    }
+    return 0;
+}
+
 int validate(const Params& p) {
     if (p.ix != 2 || p.iy != 2 || p.ilap != 2) return fail("only ix=iy=ilap=2 is supported");
     if (p.npey < 1 || p.npez < 1 || p.npy % p.npey || p.npz % p.npez)
         return fail("NPY=%d / NPZ=%d not divisible by NPEY=%d / NPEZ=%d", p.npy, p.npz, p.npey,
                     p.npez);
     if (p.npx < 3 || p.npy < 3 || p.npz < 6) return fail("grid too small");
-    if (p.nstart != 0) return fail("restart (NSTART!=0) is not supported");
+    if (p.nstart < 0) return fail("NSTART must be >= 0");
     // the reference's feature guards (src/fluxes.cpp:46-51, src/static_ic.cpp:26-29)
     if (p.lshr) return fail("unsupported feature enabled: LSHR");
     if (p.lrem) return fail("unsupported feature enabled: LREM");
@@ -556,11 +685,23 @@ r3d_host* r3d_host_create(int argc, const char* const* argv) {
     }
     h->d = derive(h->p);
     build_metrics(h);
-    if (build_state(h)) {
+    if (h->p.nstart > 0 ? restart_state(h) : build_state(h)) {
         delete h;
         return nullptr;
     }
     return h;
+}
+
+int r3d_host_scalars(const r3d_host* h, r3d_hip_scalars* out) {
+    if (!h || !out) return fail("null argument");
+    *out = h->sc;
+    return 0;
+}
+
+int r3d_host_read_gstate(const r3d_host* h, int rank, const char* path, double* const* fields,
+                         r3d_hip_scalars* s) {
+    if (!h || !path || !fields || !s) return fail("null argument");
+    return read_gstate(h, rank, path, fields, s);
 }
 
 void r3d_host_free(r3d_host* h) { delete h; }
@@ -674,7 +815,7 @@ int r3d_host_write_gstate(const r3d_host* h, int rank, const char* path,
     const int nx = p.npx + 2, ny = nry + 2, nz = nrz + 2;
     FILE* f = fopen(path, "wb");
     if (!f) return fail("cannot open dump '%s'", path);
-    const int32_t head[3] = {1129928788, 27, 14};
+    const int32_t head[3] = {kGstateMagic, 27, 14};
     const int32_t hdr[13] = {nx, ny, nz, rank, rank % p.npey, rank / p.npey, s->nit, p.lmag,
                              p.ixcon, p.iycon, p.izcon, p.itcon, p.ibcon};
     const double sc[27] = {s->dt, s->timc, s->timt, s->timi, s->umach, p.re, d.repr, d.cv,
@@ -687,9 +828,7 @@ int r3d_host_write_gstate(const r3d_host* h, int rank, const char* path,
     // gstate order: ru rv rw ro tt uu vv ww fu fv fw fr ft zru zrv zrw zro ztt
     // ww1 ww2 ww3 bx by bz zbx zby zbz; scratch arrays are not part of the
     // device state and are written as zeros
-    const int map[27] = {R3D_RU, R3D_RV, R3D_RW, R3D_RO, R3D_TT, -1, -1, -1, -1, -1, -1, -1, -1,
-                         R3D_ZRU, R3D_ZRV, R3D_ZRW, R3D_ZRO, R3D_ZTT, -1, -1, -1,
-                         R3D_BX, R3D_BY, R3D_BZ, R3D_ZBX, R3D_ZBY, R3D_ZBZ};
+    const int* map = kGstateMap;
     const int64_t cnt = (int64_t)nx * ny * nz;
     std::vector<double> zero((size_t)cnt, 0.0);
     for (int q = 0; q < 27; ++q) {
