@@ -128,6 +128,72 @@ def run_adjoint(a):
     print(json.dumps(out), flush=True)
 
 
+def run_checkpoint(a):
+    """Cost of a checkpoint: an asynchronous device snapshot (X, Z and the
+    scalars; include/r3d_hip.h) taken while the run keeps stepping, against
+    the same steps without one and against a blocking download of the same
+    fields (how the reference's dumps stop its GPU run)."""
+    pkg = load_package()
+    args = ["--npx", str(a.npx), "--npy", str(a.npy), "--npz", str(a.npz),
+            "--npey", str(a.npey), "--npez", str(a.npez)] + (MHD_FLAGS if a.mhd else [])
+    run = pkg.Run(args)
+    cells = a.npx * a.npy * a.npz
+    sim = run.make_sim()
+    run.close()
+    nf = 8 if a.mhd else 5
+    snap_bytes = 2 * nf * (a.npx + 2) * (a.npy + 2) * (a.npz + 2) * 8
+    log(f"[bench] checkpoint: {args}, snapshot of {snap_bytes / 1e9:.1f} GB")
+    sim.step(a.warmup)
+    sim.snapshot_begin()  # first use allocates the buffers
+    sim.snapshot_rank(0, ["ru"])
+    sim.sync()
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    def steps():
+        sim.step(a.steps)
+        sim.sync()
+
+    def snap_alone():
+        sim.snapshot_begin()
+        sim.snapshot_rank(0, ["ru"])
+
+    t_plain = timed(steps)
+    t_snap = timed(snap_alone)
+    t0 = time.perf_counter()
+    sim.snapshot_begin()
+    sim.step(a.steps)
+    sim.sync()
+    t_steps_with = time.perf_counter() - t0
+    sim.snapshot_rank(0, ["ru"])
+    t_total_with = time.perf_counter() - t0
+    names = pkg.PRIMARIES[:nf] + ["z" + n for n in pkg.PRIMARIES[:nf]]
+    t_block = timed(lambda: sim.download_global(names))
+    sim.close()
+    log(f"[bench] {a.steps} steps {t_plain * 1e3:.1f} ms; snapshot alone {t_snap * 1e3:.1f} ms; "
+        f"steps with a snapshot in flight {t_steps_with * 1e3:.1f} ms (snapshot done after "
+        f"{t_total_with * 1e3:.1f} ms); blocking download {t_block * 1e3:.1f} ms")
+    out = {
+        "metric": METRIC, "value": t_steps_with / t_plain - 1.0,
+        "unit": "step-time overhead of a checkpoint snapshot in flight",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": t_steps_with / a.steps * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: the reference's static polytrope with its seeded ran2 perturbation",
+        "config": {"workload": f"{'MHD' if a.mhd else 'hydro'} {a.npx}x{a.npy}x{a.npz} checkpoint",
+                   "global_cells": cells, "snapshot_bytes": snap_bytes,
+                   "steps_ms": t_plain * 1e3, "snapshot_alone_ms": t_snap * 1e3,
+                   "steps_with_snapshot_ms": t_steps_with * 1e3,
+                   "snapshot_complete_ms": t_total_with * 1e3,
+                   "blocking_download_ms": t_block * 1e3},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_ensemble(a):
     """Aggregate cell-updates/s of the batched ensemble (autotuned tiles, graph
     replay) next to the same members stepped one after another."""
@@ -203,6 +269,9 @@ def main():
                     help="time the discrete adjoint (J = final kinetic energy, checkpoints every "
                          "20 steps) against its forward run; default size 128x128x256 (2x4 "
                          "virtual ranks); the specified workload adds --mhd --steps 200")
+    ap.add_argument("--checkpoint", action="store_true",
+                    help="cost of an asynchronous checkpoint snapshot taken while stepping, "
+                         "against plain steps and a blocking download")
     ap.add_argument("--variant", type=int, default=0, choices=(0, 1),
                     help="0: bit-exact stage kernel (default); 1: FMA-contracted stage kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -228,6 +297,8 @@ def main():
         return run_ensemble(a)
     if a.adjoint:
         return run_adjoint(a)
+    if a.checkpoint:
+        return run_checkpoint(a)
     extra = MHD_FLAGS if a.mhd else []
     bpc = BYTES_PER_CELL_STAGE_MHD if a.mhd else BYTES_PER_CELL_STAGE
     cpu = None if (a.no_cpu_baseline or a.mhd) else cpu_baseline(a.npx, a.npy, a.npey, a.npez,
