@@ -56,6 +56,14 @@ class Device {
     }
     // step_gpu(...) is step(1); step_gpu(nstages, ...) is step(1, nstages).
     void step(int nsteps = 1, int nstages = 3) { check(r3d_hip_step(ctx_, nsteps, nstages)); }
+    // asynchronous checkpoint: begin returns at once; snapshot_rank waits for
+    // the copy and returns one rank as download_rank would at begin's point
+    void snapshot_begin() { check(r3d_hip_snapshot_begin(ctx_)); }
+    r3d_hip_scalars snapshot_rank(int rank, double* const fields[R3D_NFIELDS]) {
+        r3d_hip_scalars s{};
+        check(r3d_hip_snapshot_rank(ctx_, rank, fields, &s));
+        return s;
+    }
     void sync() { check(r3d_hip_sync(ctx_)); }
     r3d_hip_ctx* handle() { return ctx_; }
 
