@@ -615,15 +615,38 @@ int r3d_hip_snapshot_begin(r3d_hip_ctx* c) {
     const r3d_hip_cfg& f = c->cfg;
     const size_t n = (size_t)(f.npx + 2) * (f.npy + 2) * (f.npz + 2);
     const size_t nsf = 2 * (size_t)c->nf;  // X and Z
-    if (!c->snap_dev) {
-        if ((rc = dev_alloc(c, &c->snap_dev, nsf * n))) return rc;
+    if (!c->snap_done) {  // first use: all of these, or none
+        auto undo = [&] {
+            if (c->snap_dev) {
+                (void)hipFree(c->snap_dev);
+                c->bytes -= nsf * n * sizeof(double);
+            }
+            if (c->snap_host) (void)hipHostFree(c->snap_host);
+            if (c->snap_scal) (void)hipHostFree(c->snap_scal);
+            if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+            if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
+            c->snap_dev = c->snap_host = nullptr;
+            c->snap_scal = nullptr;
+            c->copy_stream = nullptr;
+            c->snap_ready = nullptr;
+        };
+        if ((rc = dev_alloc(c, &c->snap_dev, nsf * n))) {
+            undo();
+            return rc;
+        }
         if (hipHostMalloc((void**)&c->snap_host, nsf * n * sizeof(double)) != hipSuccess ||
-            hipHostMalloc((void**)&c->snap_scal, sizeof(DevScal)) != hipSuccess)
+            hipHostMalloc((void**)&c->snap_scal, sizeof(DevScal)) != hipSuccess) {
+            undo();
             return set_err(R3D_ENOMEM, "snapshot: pinned host allocation of %zu bytes failed",
                            nsf * n * sizeof(double));
-        HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+        }
+        if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming) != hipSuccess) {
+            undo();
+            c->snap_done = nullptr;
+            return set_err(R3D_EHIP, "snapshot: stream or event creation failed");
+        }
     }
     // the previous snapshot's copy must be done before its buffer is reused
     if (c->snap_taken) HIPCHK(hipStreamWaitEvent(c->stream, c->snap_done, 0));
