@@ -741,6 +741,9 @@ struct r3d_adj_ctx {
     std::vector<double*> ckpt;      // X at steps 0, K, 2K, ...
     std::vector<double*> segx, seg1, seg2;  // X_n, X1, X2 of the steps of one segment
     std::vector<double> segdt;
+    double* dt_pinned = nullptr;    // dt of the recomputed steps, read back once per segment
+    double* scratch = nullptr;      // a state buffer for the unneeded last output of a segment
+    bool seg_outer = false;         // segment buffers hold X0's outer z planes
     double *x0 = nullptr, *xfinal = nullptr;
     double* lam[3] = {nullptr, nullptr, nullptr};  // rotating adjoint buffers
     double* acc = nullptr;          // g_ore, g_tk, <F, lamF>, g_theta, dt's d/dRe, g_tb
@@ -790,6 +793,28 @@ int get_state(r3d_adj_ctx* a, double* dst) {
     for (int f = 0; f < a->nf; ++f)
         HIPCHK(hipMemcpyAsync(dst + f * a->fe, w->xs[w->cur][f], a->fe * sizeof(double),
                               hipMemcpyDeviceToDevice, w->stream));
+    return R3D_OK;
+}
+
+// The stage kernels write interior cells only, and the outer z planes keep
+// X0's values (RO's is rewritten by the boundary kernel every stage), so a
+// buffer that a stage writes into must hold them: copy X0's planes 0 and
+// npz+1 into every segment buffer, once per initial state.
+int seg_outer_planes(r3d_adj_ctx* a) {
+    if (a->seg_outer) return R3D_OK;
+    r3d_hip_ctx* w = a->work;
+    const size_t pp = (size_t)w->g.pp;
+    std::vector<double*> bufs(a->segx);
+    bufs.insert(bufs.end(), a->seg1.begin(), a->seg1.end());
+    bufs.insert(bufs.end(), a->seg2.begin(), a->seg2.end());
+    bufs.push_back(a->scratch);
+    for (double* buf : bufs)
+            for (int f = 0; f < a->nf; ++f)
+                for (size_t k : {(size_t)0, (size_t)w->g.npz + 1})
+                    HIPCHK(hipMemcpyAsync(buf + f * a->fe + k * pp, a->x0 + f * a->fe + k * pp,
+                                          pp * sizeof(double), hipMemcpyDeviceToDevice,
+                                          w->stream));
+    a->seg_outer = true;
     return R3D_OK;
 }
 
@@ -911,6 +936,9 @@ int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** o
             if ((rc = alloc_state(a, &p))) return fail(rc);
     }
     a->segdt.assign(a->K, 0.0);
+    if ((rc = alloc_state(a, &a->scratch))) return fail(rc);
+    if (hipHostMalloc((void**)&a->dt_pinned, a->K * sizeof(double)) != hipSuccess)
+        return fail(set_err(R3D_ENOMEM, "adjoint: pinned dt buffer"));
     // ghost entries of the initial state and the adjoint fields stay zero
     for (double* p : {a->x0, a->xfinal, a->lam[0], a->lam[1], a->lam[2]})
         if (hipMemset(p, 0, a->fe * a->nf * sizeof(double)) != hipSuccess)
@@ -923,8 +951,9 @@ int r3d_adj_destroy(r3d_adj_ctx* a) {
     if (!a) return R3D_OK;
     if (a->work) (void)hipStreamSynchronize(a->work->stream);
     for (double* p : {a->x0, a->xfinal, a->lam[0], a->lam[1], a->lam[2], a->acc, a->part,
-                      a->bval})
+                      a->bval, a->scratch})
         if (p) (void)hipFree(p);
+    if (a->dt_pinned) (void)hipHostFree(a->dt_pinned);
     if (a->bkey) (void)hipFree(a->bkey);
     for (auto* v : {&a->ckpt, &a->segx, &a->seg1, &a->seg2})
         for (double* p : *v)
@@ -947,6 +976,7 @@ int r3d_adj_set_initial(r3d_adj_ctx* a, const double* const* fields) {
         if (rc) return rc;
     }
     a->nsteps = 0;
+    a->seg_outer = false;
     return R3D_OK;
 }
 
@@ -996,20 +1026,37 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
     const double g1 = C.gam[0], g2 = C.gam[1], g3 = C.gam[2];
     const double z1 = C.zeta[0], z2 = C.zeta[1];
     const int N = a->nsteps, nseg = (N + a->K - 1) / a->K;
+    if ((rc = seg_outer_planes(a))) return rc;
     for (int sgm = nseg - 1; sgm >= 0; --sgm) {
         const int n0 = sgm * a->K, n1 = std::min(N, n0 + a->K);
-        // X_n, X1, X2 and dt of every step of this segment, from its checkpoint
-        if ((rc = put_state(a, a->ckpt[sgm]))) return rc;
+        // X_n, X1, X2 and dt of every step of this segment, from its
+        // checkpoint.  The stages read and write the checkpoint and segment
+        // buffers directly (the work context's state pointers are redirected,
+        // then restored), and dt comes back by one copy per segment.
+        double* own[2][NXF];
+        std::memcpy(own, w->xs, sizeof own);
+        w->cur = 0;
+        for (int f = 0; f < a->nf; ++f) w->xs[0][f] = a->ckpt[sgm] + f * a->fe;
+        w->next_valid = false;
         for (int n = n0; n < n1; ++n) {
-            if ((rc = get_state(a, a->segx[n - n0]))) return rc;
-            double* keep[2] = {a->seg1[n - n0], a->seg2[n - n0]};
-            if ((rc = step_one(w, 3, keep))) return rc;
-            r3d_hip_scalars sc{};
-            if ((rc = r3d_hip_get_scalars(w, &sc))) return rc;
-            a->segdt[n - n0] = sc.dt;
+            // the last step's X_{n+1} (a checkpoint or the final state) is
+            // stored already; it goes to a scratch buffer
+            double* outs[3] = {a->seg1[n - n0], a->seg2[n - n0],
+                               n + 1 < n1 ? a->segx[n + 1 - n0] : a->scratch};
+            rc = step_one(w, 3, nullptr, outs);
+            if (!rc && hipMemcpyAsync(a->dt_pinned + (n - n0), &w->scal->dt, sizeof(double),
+                                      hipMemcpyDeviceToHost, w->stream) != hipSuccess)
+                rc = set_err(R3D_EHIP, "dt readback failed");
+            if (rc) break;
         }
+        std::memcpy(w->xs, own, sizeof own);
+        w->cur = 0;
+        w->next_valid = false;
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(w->stream));
+        for (int n = n0; n < n1; ++n) a->segdt[n - n0] = a->dt_pinned[n - n0];
         for (int n = n1 - 1; n >= n0; --n) {
-            const double* xn = a->segx[n - n0];
+            const double* xn = n == n0 ? a->ckpt[sgm] : a->segx[n - n0];
             const double* x1b = a->seg1[n - n0];
             const double* x2b = a->seg2[n - n0];
             const double* xn1 =

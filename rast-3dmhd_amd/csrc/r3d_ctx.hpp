@@ -95,7 +95,11 @@ int check_ready(r3d_hip_ctx* c);
 // One step of nstages stages (the body of r3d_hip_step).  keep[s] (s = 0, 1),
 // if given, receives the state after stage s+1 and its boundary pass (nf
 // fields at a stride of field_elems).
-int step_one(r3d_hip_ctx* c, int nstages, double* const* keep);
+// One step.  keep[s] (s < 2, may be null): copy stage s's output there.
+// outs[s] (may be null): stage s writes its output into that state buffer
+// (nf fields of field_elems each) instead of the context's own; the caller
+// restores c->xs afterwards.
+int step_one(r3d_hip_ctx* c, int nstages, double* const* keep, double* const* outs = nullptr);
 // The FMA-contracted stage kernel (r3d_fma.hip); ty must be the run's tile rows.
 hipError_t launch_stage_fma(const StageArgs& a, const Consts& C, const Metrics& M, int ty,
                             bool mag, unsigned grid, hipStream_t stream);

@@ -319,7 +319,7 @@ int r3d::check_ready(r3d_hip_ctx* c) {
     return R3D_OK;
 }
 
-int r3d::step_one(r3d_hip_ctx* c, int nstages, double* const* keep) {
+int r3d::step_one(r3d_hip_ctx* c, int nstages, double* const* keep, double* const* outs) {
     int rc;
     if (c->cfg.tc != 0.0) {
         // time-dependent plume temperature (src/bcon.cpp:45) from the host libm
@@ -334,6 +334,8 @@ int r3d::step_one(r3d_hip_ctx* c, int nstages, double* const* keep) {
     for (int s = 0; s < nstages; ++s) {
         const int last = s == nstages - 1;
         const int zmode = !last ? 0 : (s == 0 ? 1 : 2);
+        if (outs && outs[s])
+            for (int f = 0; f < c->nf; ++f) c->xs[c->cur ^ 1][f] = outs[s] + f * fe;
         if ((rc = launch_stage(c, s, zmode, nstages == 3 && s == 2))) return rc;
         if (keep && s < 2 && keep[s])
             for (int f = 0; f < c->nf; ++f)
