@@ -50,7 +50,9 @@ typedef struct r3d_hip_cfg {
     int npx, npy, npz;          /* global interior points */
     int npey, npez;             /* virtual rank grid (npy%npey == 0, npz%npez == 0) */
     int lmag, lrot;             /* physics switches */
-    int itcon, izcon, ibcon;    /* boundary selectors (reference src/bcon.cpp) */
+    int itcon, izcon, ibcon;    /* boundary selectors (reference src/bcon.cpp); itcon 2 also
+                                   adds the Fortran's embedded heat loss to the energy flux
+                                   (bit-exact variant only), itcon 3 gives R3D_EUNSUP */
     int variant;                /* 0 = bit-exact (no FMA contraction), 1 = FMA-contracted */
     int device;                 /* HIP device ordinal */
     double c13, c23, c43;
@@ -61,6 +63,8 @@ typedef struct r3d_hip_cfg {
     double tp, tc, hh, xp, yp;  /* plume boundary condition */
     double dzf;                 /* f4(1/(npz-1)), the BCON flux spacing numerator */
     double tb;                  /* bottom temperature from the static profile */
+    double zp, qfh;             /* ITCON=2 embedded heat loss: centre z (with xp, yp, hh, tp)
+                                   and ramp time scale (golden/fortran/3dmhdsub.f:1797-1822) */
 } r3d_hip_cfg;
 
 /* Global 1-D metrics.  x arrays have npx+2 entries, y arrays npy+2, z arrays

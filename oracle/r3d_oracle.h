@@ -25,6 +25,12 @@
  *                            (src/fluxes.cpp:539-541); the oracle restates the
  *                            Fortran (golden/fortran/3dmhdsub.f) and is checked
  *                            only by invariants (B=0 reduces to hydro).
+ *   ITCON=2 heat loss ...... parity unpinned: the reference C++ refuses it
+ *                            (src/fluxes.cpp:50); the term restates
+ *                            golden/fortran/3dmhdsub.f:1797-1822 and is checked
+ *                            by closed forms (Gaussian sink of FWHM HH on a
+ *                            fluid at rest, exact 1/RO scaling, the TANH ramp
+ *                            at 1/10, TP=0 equal to the pinned ITCON=0 run).
  */
 #ifndef R3D_ORACLE_H
 #define R3D_ORACLE_H

@@ -55,6 +55,9 @@ struct Consts {
     double theta, tb, hh, tpr, tp;
     double gam[3], zeta[2];
     double dtfix;  // > 0: every step takes this dt instead of the CFL one (adjoint runs)
+    // ITCON=2 embedded heat loss amplitude TP*OCV*ramp(TIMT) of this step
+    // (golden/fortran/3dmhdsub.f:1799-1822); read by the HEAT stage kernels only
+    double hloss;
     int lrot, lmag, top_mode, half_t, izcon, bz_zero;
     int mag_bot;  // LMAG one-sided z stencils on the bottom wall (npez > 1,
                   // golden/fortran/3dmhdsub.f:1126, 1392, 1513 sit in an ELSE IF)
@@ -66,6 +69,8 @@ struct Metrics {
     const double *dyy, *d2yy, *ddy;
     const double *dzz, *d2zz, *ddz, *rkapa, *dkapa;
     const double *e1, *e2;  // plume Gaussians exp(...) per x / per y (host libm)
+    // ITCON=2 heat-loss Gaussians exp(CLN*(EXX-XP)**2), ... per x / y / z (host libm)
+    const double *hgx, *hgy, *hgz;
     double dz_top, dz_bot;  // f4(1/(npz-1)) / dzzdz at the boundary planes
 };
 

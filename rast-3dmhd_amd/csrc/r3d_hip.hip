@@ -109,6 +109,15 @@ int r3d::dev_alloc(r3d_hip_ctx* c, double** p, size_t n) {
 
 namespace {
 
+// ITCON=2 heat-loss amplitude at time timt: TP*OCV, ramped by
+// 0.5*(1+TANH(LOG(3.0E00)/QFH*(TIMT-TC))) when TC != 0, with the Fortran's
+// single-precision LOG(3.0E00) (golden/fortran/3dmhdsub.f:1801-1820)
+double heat_amplitude(const r3d_hip_cfg& f, double timt) {
+    if (f.itcon != 2) return 0.0;
+    if (f.tc == 0.0) return f.tp * f.ocv;
+    return f.tp * f.ocv * 0.5 * (1.0 + tanh((double)logf(3.0f) / f.qfh * (timt - f.tc)));
+}
+
 void fill_consts(r3d_hip_ctx* c) {
     const r3d_hip_cfg& f = c->cfg;
     Consts& C = c->C;
@@ -145,6 +154,7 @@ void fill_consts(r3d_hip_ctx* c) {
     C.hh = f.hh;
     C.tp = f.tp;
     C.tpr = f.tp;
+    C.hloss = heat_amplitude(f, 0.0);
     C.gam[0] = f.gam1;
     C.gam[1] = f.gam2;
     C.gam[2] = f.gam3;
@@ -241,6 +251,17 @@ int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
         if (c->cfg.variant == 1) {
             fma_err = launch_stage_fma(a, c->C, c->M, c->ty, c->cfg.lmag != 0, c->grid,
                                        c->stream);
+        } else if (c->cfg.itcon == 2) {
+#define R3D_STAGE(TY, S, Z, MG) \
+    hipLaunchKernelGGL((k_stage<TY, S, Z, MG, true>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
+            if (c->cfg.lmag) {
+                R3D_STAGES(TILE_Y_MAG, true)
+            } else {
+                R3D_STAGES(8, false)
+            }
+#undef R3D_STAGE
+#define R3D_STAGE(TY, S, Z, MG) \
+    hipLaunchKernelGGL((k_stage<TY, S, Z, MG>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
         } else if (c->cfg.lmag) {
             R3D_STAGES(TILE_Y_MAG, true)
         } else {
@@ -328,6 +349,7 @@ int r3d::step_one(r3d_hip_ctx* c, int nstages, double* const* keep, double* cons
         HIPCHK(hipStreamSynchronize(c->stream));
         const double timt = d.timc + d.timi;
         c->C.tpr = c->cfg.tp + (1.0 - c->cfg.tp) * exp(-timt / c->cfg.tc);
+        c->C.hloss = heat_amplitude(c->cfg, timt);
     }
     if (!c->next_valid && (rc = launch_dt(c))) return rc;
     const size_t fe = field_elems(c->g);
@@ -367,8 +389,11 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
                        f.npy, f.npz);
     if (f.npz / f.npez < 3) return set_err(R3D_EINVAL, "each virtual z rank needs >= 3 planes");
     if (f.izcon != 0 && f.izcon != 1) return set_err(R3D_EINVAL, "BCON: invalid IZCON %d", f.izcon);
-    if (f.itcon == 2 || f.itcon == 3)
-        return set_err(R3D_EUNSUP, "FLUXES: unsupported feature enabled: ITCON embedded heat/thermal");
+    // ITCON=3 is only named in the Fortran (3dmhd.f:645, 1179), never implemented
+    if (f.itcon == 3)
+        return set_err(R3D_EUNSUP, "FLUXES: unsupported feature enabled: ITCON=3 embedded thermal");
+    if (f.itcon == 2 && f.variant != 0)
+        return set_err(R3D_EUNSUP, "ITCON=2 heat loss is built in the bit-exact variant only");
     if (f.variant != 0 && f.variant != 1)
         return set_err(R3D_EUNSUP, "variant %d: 0 (bit-exact) and 1 (FMA) are built", f.variant);
     HIPCHK(hipSetDevice(f.device));
@@ -390,7 +415,7 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     fill_consts(c);
 
     // stage tiling: 64 x ty columns, z chunks sized for >= ~8 blocks per CU
-    c->ty = f.lmag ? TILE_Y_MAG : hydro_tile_rows(f.npy);
+    c->ty = f.lmag ? TILE_Y_MAG : f.itcon == 2 ? 8 : hydro_tile_rows(f.npy);
     c->ntx = (f.npx + 63) / 64;
     c->nty = (f.npy + c->ty - 1) / c->ty;
     c->kc = 8;
@@ -433,7 +458,7 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
             return fail(set_err(R3D_EHIP, "hipMemcpy scalars failed"));
     }
     const size_t nmet = 3 * (size_t)(f.npx + 2) + 3 * (size_t)(f.npy + 2) + 5 * (size_t)(f.npz + 2) +
-                        (size_t)(f.npx + 2) + (size_t)(f.npy + 2);
+                        2 * ((size_t)(f.npx + 2) + (size_t)(f.npy + 2)) + (size_t)(f.npz + 2);
     if ((rc = dev_alloc(c, &c->metric_buf, nmet))) return fail(rc);
     const size_t stage_elems = (size_t)std::max<long>((long)c->nx * c->ny * c->nz,
                                                       (long)(f.npx + 2) * (f.npy + 2));
@@ -493,8 +518,20 @@ int r3d_hip_set_metrics(r3d_hip_ctx* c, const r3d_hip_metrics* m) {
         const double dy = m->wyy[j] - f.yp;
         e2[j] = c->C.top_mode == TOP_GAUSS_T ? exp(cln * dy * dy) : exp(cln * (dy * dy));
     }
+    // ITCON=2 heat-loss Gaussians, CLN=-4.0E00*LOG(2.0E00)/HH/HH and
+    // EXP(CLN*(EXX(I)-XP)**2) as golden/fortran/3dmhdsub.f:1800-1820 evaluates them
+    std::vector<double> hg(X + Y + Z, 0.0);
+    if (f.itcon == 2) {
+        if (!m->zee) return set_err(R3D_EINVAL, "metric array zee missing (ITCON=2)");
+        const double hcln = (double)(-4.0f * logf(2.0f)) / f.hh / f.hh;
+        for (int i = 0; i < X; ++i) hg[i] = exp(hcln * ((m->exx[i] - f.xp) * (m->exx[i] - f.xp)));
+        for (int j = 0; j < Y; ++j)
+            hg[X + j] = exp(hcln * ((m->wyy[j] - f.yp) * (m->wyy[j] - f.yp)));
+        for (int k = 0; k < Z; ++k)
+            hg[X + Y + k] = exp(hcln * ((m->zee[k] - f.zp) * (m->zee[k] - f.zp)));
+    }
     std::vector<double> h;
-    h.reserve(3 * X + 3 * Y + 5 * Z + X + Y);
+    h.reserve(3 * X + 3 * Y + 5 * Z + 2 * (X + Y) + Z);
     auto put = [&](const double* p, int n) -> const double* {
         const size_t off = h.size();
         h.insert(h.end(), p, p + n);
@@ -514,6 +551,9 @@ int r3d_hip_set_metrics(r3d_hip_ctx* c, const r3d_hip_metrics* m) {
     M.dkapa = put(m->dkapa, Z);
     M.e1 = put(e1.data(), X);
     M.e2 = put(e2.data(), Y);
+    M.hgx = put(hg.data(), X);
+    M.hgy = put(hg.data() + X, Y);
+    M.hgz = put(hg.data() + X + Y, Z);
     M.dz_top = f.dzf / m->dzzdz[1];
     M.dz_bot = f.dzf / m->dzzdz[f.npz];
     HIPCHK(hipMemcpyAsync(c->metric_buf, h.data(), h.size() * sizeof(double),

@@ -205,7 +205,9 @@ __device__ __forceinline__ void block_partials(double pm[NPART], double (*red)[N
 // One tile t (0 <= t < a.nblocks) of one RK stage; the caller maps blocks to
 // tiles.  ZMODE: 0 = Z' is the next stage's mix, 1 = Z keeps the saved state
 // (a step truncated after stage 1), 2 = Z is not written (last stage).
-template <int TY, int STAGE, int ZMODE, bool MAG>
+// HEAT: the ITCON=2 embedded heat loss term is added to FT (a separate
+// instantiation, so that the default kernels carry none of its registers).
+template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, const Metrics& M,
                                            const int t) {
     constexpr int ROWS = TY + 2;
@@ -292,6 +294,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // x/y metric factors of this column (src/fluxes.cpp:90, 250-254)
     const double dxx = M.dxx[li], d2xx = M.d2xx[li];
     const double dyy = M.dyy[lj], d2yy = M.d2yy[lj];
+    const double hgx = HEAT ? M.hgx[li] : 0.0, hgy = HEAT ? M.hgy[lj] : 0.0;
     if (STAGE == 2 && heavy) {
         ddxy_s[r][c] = dmin(M.ddx[li], M.ddy[lj]);
         pm_s[0][r - 1][ln] = 0.0;
@@ -519,6 +522,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 ft = ft + ((ttN - ttS) * tmpy1 + (ttN - t2c + ttS) * tmpy2) * cc.ri * C.tk * rk;
                 ft = ft + ((ttU - ttD) * tmpz1 + (ttU - t2c + ttD) * tmpz2) * cc.ri * C.tk * rk;
                 ft = ft + (ttU - ttD) * C.hz * dzz * cc.ri * C.tk * dk;
+                if constexpr (HEAT) {
+                    // localized embedded heat loss, golden/fortran/3dmhdsub.f:1799-1822
+                    // (RO there already holds 1/RO, :1405), same operation order
+                    const double hgz = *(const gdouble*)(M.hgz + k);
+                    ft = ft - C.hloss * cc.ri * hgx / C.hh * hgy / C.hh * hgz / C.hh;
+                }
                 const double ux = (SH(SH_U, r, c + 1) - SH(SH_U, r, c - 1)) * hxd * dxx;
                 const double vx = (SH(SH_V, r, c + 1) - SH(SH_V, r, c - 1)) * hxd * dxx;
                 const double wx = (SH(SH_W, r, c + 1) - SH(SH_W, r, c - 1)) * hxd * dxx;
@@ -674,12 +683,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
 }
 
 // One RK stage of one run: block -> tile with the XCD-aware order.
-template <int TY, int STAGE, int ZMODE, bool MAG>
+template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false>
 __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, const Consts C,
                                                          const Metrics M) {
     const int t = logical_block();
     if (t >= a.nblocks) return;
-    stage_tile<TY, STAGE, ZMODE, MAG>(a, C, M, t);
+    stage_tile<TY, STAGE, ZMODE, MAG, HEAT>(a, C, M, t);
 }
 
 }  // namespace R3D_KNS

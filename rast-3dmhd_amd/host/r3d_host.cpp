@@ -723,7 +723,9 @@ int validate(const Params& p) {
     if (p.lrem) return fail("unsupported feature enabled: LREM");
     if (p.rlax > 0.0) return fail("unsupported feature enabled: flux relaxation (RLAX)");
     if (p.id != 0) return fail("unsupported feature enabled: density diffusion ID");
-    if (p.itcon == 2 || p.itcon == 3) return fail("unsupported feature enabled: ITCON embedded heat/thermal");
+    // ITCON=2 (embedded heat loss, golden/fortran/3dmhdsub.f:1797-1822) is
+    // built; ITCON=3 has only a banner line in the Fortran (3dmhd.f:645, 1179)
+    if (p.itcon == 3) return fail("unsupported feature enabled: ITCON=3 embedded thermal");
     if (p.ixcon != 0 || p.iycon != 0) return fail("unsupported feature enabled: non-periodic horizontal boundaries");
     if (p.lmag && p.ntube != 0) return fail("unsupported feature enabled: TUBE models");
     if (p.ampb != 0.0 && !p.lmag) return fail("AMPB magnetic layer requires LMAG");
@@ -818,6 +820,8 @@ int r3d_host_cfg(const r3d_host* h, r3d_hip_cfg* o) {
     o->yp = p.yp;
     o->dzf = single(1.0f / (float)(p.npz - 1));
     o->tb = h->tb;
+    o->zp = p.zp;
+    o->qfh = p.qfh;
     return 0;
 }
 
