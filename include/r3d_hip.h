@@ -65,6 +65,9 @@ typedef struct r3d_hip_cfg {
     double tb;                  /* bottom temperature from the static profile */
     double zp, qfh;             /* ITCON=2 embedded heat loss: centre z (with xp, yp, hh, tp)
                                    and ramp time scale (golden/fortran/3dmhdsub.f:1797-1822) */
+    double dh, dv, zmax;        /* density diffusion coefficients and domain depth */
+    int id;                     /* density diffusion: 0 off, 1 WWZ = 1/ROM, 2 wall Gaussians
+                                   (golden/fortran/3dmhdsub.f:1306-1380; needs NGRID=0) */
 } r3d_hip_cfg;
 
 /* Global 1-D metrics.  x arrays have npx+2 entries, y arrays npy+2, z arrays

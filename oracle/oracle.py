@@ -43,6 +43,12 @@ def lib():
         L.o_fluxes.restype = ctypes.c_int
         L.o_bcon.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.o_bcon.restype = ctypes.c_int
+        L.o_density_prep.argtypes = [ctypes.c_void_p]
+        L.o_density_prep.restype = ctypes.c_int
+        L.o_hmean.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.o_hmean.restype = ctypes.c_int
+        L.o_rom.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.o_rom.restype = ctypes.POINTER(ctypes.c_double)
         L.o_communicate.argtypes = [ctypes.c_void_p]
         L.o_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.o_field.restype = ctypes.POINTER(ctypes.c_double)
@@ -113,6 +119,16 @@ class Sim:
 
     def bcon(self, rank):
         self._check(lib().o_bcon(self.h, rank))
+
+    def density_prep(self):
+        self._check(lib().o_density_prep(self.h))
+
+    def hmean(self, name):
+        """Horizontal mean of a field per local plane, into rom()."""
+        self._check(lib().o_hmean(self.h, FIELDS.index(name)))
+
+    def rom(self, rank):
+        return np.ctypeslib.as_array(lib().o_rom(self.h, rank), shape=(self.nz,))
 
     def communicate(self):
         lib().o_communicate(self.h)

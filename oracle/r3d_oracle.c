@@ -449,6 +449,7 @@ typedef struct {
     int mype, mypey, mypez;
     double* f[NF3];
     double* m[NM1];
+    double* rom; /* horizontal mean of RO per local plane (density diffusion) */
 } o_rank;
 
 struct o_sim {
@@ -493,6 +494,7 @@ o_sim* o_create(int argc, const char* const* argv) {
         const int len[NM1] = {s->d.nx, s->d.nx, s->d.nx, s->d.nx, s->d.ny, s->d.ny, s->d.ny,
                               s->d.ny, s->d.nz, s->d.nz, s->d.nz, s->d.nz, s->d.nz, s->d.nz};
         for (int m = 0; m < NM1; ++m) R->m[m] = calloc((size_t)len[m], sizeof(double));
+        R->rom = calloc((size_t)s->d.nz, sizeof(double));
     }
     s->tb = 1.0;
     return s;
@@ -503,6 +505,7 @@ void o_free(o_sim* s) {
     for (int r = 0; r < s->nranks; ++r) {
         for (int f = 0; f < NF3; ++f) free(s->r[r].f[f]);
         for (int m = 0; m < NM1; ++m) free(s->r[r].m[m]);
+        free(s->r[r].rom);
     }
     free(s->r);
     free(s);
@@ -516,6 +519,10 @@ void o_dims(const o_sim* s, int dims[10]) {
 double* o_field(o_sim* s, int rank, int fid) {
     if (rank < 0 || rank >= s->nranks || fid < 0 || fid >= NF3) return NULL;
     return s->r[rank].f[fid];
+}
+double* o_rom(o_sim* s, int rank) {
+    if (rank < 0 || rank >= s->nranks) return NULL;
+    return s->r[rank].rom;
 }
 double* o_metric(o_sim* s, int rank, int mid) {
     if (rank < 0 || rank >= s->nranks || mid < 0 || mid >= NM1) return NULL;

@@ -31,6 +31,16 @@
  *                            by closed forms (Gaussian sink of FWHM HH on a
  *                            fluid at rest, exact 1/RO scaling, the TANH ramp
  *                            at 1/10, TP=0 equal to the pinned ITCON=0 run).
+ *   horizontal mean ........ bit-identical to the reference's own
+ *                            horizontal_mean (src/communicate.cpp:184-223) run
+ *                            from its sources (oracle/_ref/hmean, several
+ *                            decompositions)
+ *   density diffusion ID ... parity unpinned: the reference C++ refuses ID
+ *                            (src/fluxes.cpp:49); the terms restate
+ *                            golden/fortran/3dmhdsub.f:1306-1380 and are checked
+ *                            by discrete-Laplacian closed forms (cosine modes,
+ *                            quadratic z profiles, the mirrored wall planes,
+ *                            1/ROM and the ID=2 wall profile).
  */
 #ifndef R3D_ORACLE_H
 #define R3D_ORACLE_H
@@ -59,6 +69,12 @@ int o_step(o_sim* s, int nstages);
 /* Standalone pieces for invariant tests. */
 int o_fluxes(o_sim* s, int rank);      /* src/fluxes.cpp:44 */
 int o_bcon(o_sim* s, int rank);        /* src/bcon.cpp:12 */
+/* Density diffusion (ID != 0): mirror RO's outer planes, then o_hmean of RO
+ * (golden/fortran/3dmhdsub.f:1311-1328).  o_step calls it before FLUXES. */
+int o_density_prep(o_sim* s);
+/* Horizontal mean of field fid into every rank's o_rom() (src/communicate.cpp:184). */
+int o_hmean(o_sim* s, int fid);
+double* o_rom(o_sim* s, int rank);
 void o_communicate(o_sim* s);          /* src/communicate.cpp:171 */
 
 /* Introspection.  dims = {nx, ny, nz, npey, npez, nranks, npx, npy, npz, lmag}. */

@@ -19,7 +19,7 @@ CXXFLAGS := -O2 -std=c++17 -ffp-contract=off -fopenmp -I$(REF)/include
 SRCS := params topology grid ode ran2 static_ic fluxes bcon communicate step
 OBJS := $(addprefix $(OUT)/obj/,$(addsuffix .o,$(SRCS)))
 
-all: $(OUT)/3dmhd $(OUT)/3dmhd_nodump
+all: $(OUT)/3dmhd $(OUT)/3dmhd_nodump $(OUT)/hmean
 
 $(OUT)/obj/%.o: $(REF)/src/%.cpp
 	@mkdir -p $(OUT)/obj
@@ -40,3 +40,11 @@ $(OUT)/3dmhd_nodump: $(OUT)/obj/main_nodump.o $(OBJS)
 	$(MPICXX) $(CXXFLAGS) -o $@ $^ -lm
 
 .PHONY: all
+
+# the reference's horizontal_mean alone, driven by oracle/ref_hmean.cpp
+$(OUT)/obj/ref_hmean.o: $(HERE)/ref_hmean.cpp
+	@mkdir -p $(OUT)/obj
+	$(MPICXX) $(CXXFLAGS) -c -o $@ $<
+
+$(OUT)/hmean: $(OUT)/obj/ref_hmean.o $(OBJS)
+	$(MPICXX) $(CXXFLAGS) -o $@ $^ -lm

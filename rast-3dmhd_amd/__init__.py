@@ -42,7 +42,8 @@ class Cfg(ctypes.Structure):
                  "hx", "h2x", "hy", "h2y", "hz", "h2z",
                  "re", "repr", "ore", "cv", "ocv", "gamma", "grav", "theta", "sff",
                  "rm", "orm", "obeta", "omx", "omz",
-                 "tp", "tc", "hh", "xp", "yp", "dzf", "tb", "zp", "qfh")]
+                 "tp", "tc", "hh", "xp", "yp", "dzf", "tb", "zp", "qfh", "dh", "dv", "zmax")] + \
+               [("id", ctypes.c_int)]
 
 
 METRIC_NAMES = ["exx", "dxxdx", "d2xxdx2", "ddx", "wyy", "dyydy", "d2yydy2", "ddy",

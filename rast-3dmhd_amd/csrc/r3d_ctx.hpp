@@ -52,6 +52,9 @@ struct r3d_hip_ctx {
     double* partial = nullptr;
     r3d::DevScal* scal = nullptr;
     double* metric_buf = nullptr;
+    // density diffusion: per-(plane, y rank) partial means, then ROM, the two
+    // coefficient rows and the ID=2 wall profile, (npz+2) each
+    double* den_buf = nullptr;
     double* stage_buf = nullptr;
     double* gstage = nullptr;  // one global field, (npx+2)(npy+2)(npz+2), allocated on first use
     // asynchronous snapshot (r3d_hip_snapshot_begin): X and Z as dense global

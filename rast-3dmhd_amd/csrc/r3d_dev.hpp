@@ -58,6 +58,9 @@ struct Consts {
     // ITCON=2 embedded heat loss amplitude TP*OCV*ramp(TIMT) of this step
     // (golden/fortran/3dmhdsub.f:1799-1822); read by the HEAT stage kernels only
     double hloss;
+    // density diffusion (ID != 0): the horizontal and vertical terms are on
+    // (DH != 0, DV != 0); read by the DEN stage kernels only
+    int den_h, den_v;
     int lrot, lmag, top_mode, half_t, izcon, bz_zero;
     int mag_bot;  // LMAG one-sided z stencils on the bottom wall (npez > 1,
                   // golden/fortran/3dmhdsub.f:1126, 1392, 1513 sit in an ELSE IF)
@@ -71,6 +74,9 @@ struct Metrics {
     const double *e1, *e2;  // plume Gaussians exp(...) per x / per y (host libm)
     // ITCON=2 heat-loss Gaussians exp(CLN*(EXX-XP)**2), ... per x / y / z (host libm)
     const double *hgx, *hgy, *hgz;
+    // density diffusion per global plane (0..npz+1), rewritten before every
+    // stage: ROM (horizontal mean of RO), DH*ORE*WWZ and DV*ORE*WWZ
+    const double *rom, *dth, *dtv;
     double dz_top, dz_bot;  // f4(1/(npz-1)) / dzzdz at the boundary planes
 };
 

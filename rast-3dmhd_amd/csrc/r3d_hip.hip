@@ -94,6 +94,66 @@ __global__ __launch_bounds__(HALO_THREADS) void k_halo(const Geo g, const State 
     halo_thread(g, x, nf, blockIdx.x * HALO_THREADS + threadIdx.x, 1 + blockIdx.y);
 }
 
+// Density diffusion, before each stage (golden/fortran/3dmhdsub.f:1311-1328).
+// The outer RO planes take their mirror points, RO(:,:,1) = RO(:,:,3) and
+// RO(:,:,NZ) = RO(:,:,NZ-2), over the whole plane: the x/y ghost cells of
+// the source plane are its periodic images (the reference's halo exchange
+// keeps them current; here they are read through the wrap).
+constexpr int DEN_THREADS = 256;
+__global__ __launch_bounds__(DEN_THREADS) void k_den_mirror(const Geo g, double* ro) {
+    const int W = g.npx + 2, H = g.npy + 2;
+    const int q = blockIdx.x * DEN_THREADS + threadIdx.x;
+    if (q >= W * H) return;
+    const int i = q % W, j = q / W;
+    const int si = i == 0 ? g.npx : (i == g.npx + 1 ? 1 : i);
+    const int sj = j == 0 ? g.npy : (j == g.npy + 1 ? 1 : j);
+    ro[g.at(i, j, 0)] = ro[g.at(si, sj, 2)];
+    ro[g.at(i, j, g.npz + 1)] = ro[g.at(si, sj, g.npz - 1)];
+}
+
+// Horizontal mean, first half (src/communicate.cpp:193-199): thread
+// (K, q) sums y rank q's interior rows of global plane K, j outer and i
+// inner, in one sequential chain, and divides by npx*npy.  The loads of a
+// row run ahead of the additions, which keep their order.
+__global__ __launch_bounds__(DEN_THREADS) void k_den_partial(const Geo g, const double* ro,
+                                                             double* part, int npey, double cnt) {
+    const int t = blockIdx.x * DEN_THREADS + threadIdx.x;
+    if (t >= (g.npz + 2) * npey) return;
+    const int K = t / npey, q = t % npey, nry = g.npy / npey;
+    constexpr int U = 16;
+    double sum = 0.0;
+    for (int j = q * nry + 1; j <= (q + 1) * nry; ++j) {
+        const double* row = ro + g.at(1, j, K);
+        int i = 0;
+        for (; i + U <= g.npx; i += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = row[i + u];
+#pragma unroll
+            for (int u = 0; u < U; ++u) sum += v[u];
+        }
+        for (; i < g.npx; ++i) sum += row[i];
+    }
+    part[t] = sum / cnt;
+}
+
+// Second half (src/communicate.cpp:205-211): the y ranks' partial means
+// added in rank order; then the per-plane coefficients TMP = DH*ORE*WWZ(K)
+// and DV*ORE*WWZ(K), WWZ = 1/ROM (ID=1) or the wall profile (ID=2).
+__global__ __launch_bounds__(DEN_THREADS) void k_den_final(const double* part, double* rom,
+                                                           double* dth, double* dtv,
+                                                           const double* wwz2, int nk, int npey,
+                                                           int id, double dhore, double dvore) {
+    const int K = blockIdx.x * DEN_THREADS + threadIdx.x;
+    if (K >= nk) return;
+    double acc = part[(long)K * npey];
+    for (int q = 1; q < npey; ++q) acc += part[(long)K * npey + q];
+    rom[K] = acc;
+    const double wz = id == 1 ? 1.0 / acc : wwz2[K];
+    dth[K] = dhore * wz;
+    dtv[K] = dvore * wz;
+}
+
 }  // namespace
 
 int r3d::dev_alloc(r3d_hip_ctx* c, double** p, size_t n) {
@@ -155,6 +215,8 @@ void fill_consts(r3d_hip_ctx* c) {
     C.tp = f.tp;
     C.tpr = f.tp;
     C.hloss = heat_amplitude(f, 0.0);
+    C.den_h = f.id != 0 && f.dh != 0.0;
+    C.den_v = f.id != 0 && f.dv != 0.0;
     C.gam[0] = f.gam1;
     C.gam[1] = f.gam2;
     C.gam[2] = f.gam3;
@@ -235,44 +297,65 @@ StageArgs stage_args(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     return a;
 }
 
+// The five (stage, z mode) kernels of one instantiation family.
+template <int TY, bool MG, bool HT = false, bool DN = false>
+void launch_stage_kernel(r3d_hip_ctx* c, const StageArgs& a, int stage, int zmode) {
+    const dim3 blk((TY + 3) * 64), grid(c->grid);
+#define R3D_STAGE(S, Z) \
+    hipLaunchKernelGGL((k_stage<TY, S, Z, MG, HT, DN>), grid, blk, 0, c->stream, a, c->C, c->M)
+    if (stage == 0 && zmode == 0) R3D_STAGE(0, 0);
+    else if (stage == 0) R3D_STAGE(0, 1);
+    else if (stage == 1 && zmode == 0) R3D_STAGE(1, 0);
+    else if (stage == 1) R3D_STAGE(1, 2);
+    else R3D_STAGE(2, 2);
+#undef R3D_STAGE
+}
+
+// Density diffusion set-up of this stage's input: mirrored outer RO planes,
+// horizontal mean ROM and the per-plane coefficients.
+int launch_density_prep(r3d_hip_ctx* c) {
+    const r3d_hip_cfg& f = c->cfg;
+    double* ro = c->xs[c->cur][QRO];
+    const int nk = f.npz + 2, cells = (f.npx + 2) * (f.npy + 2);
+    hipLaunchKernelGGL(k_den_mirror, dim3((cells + DEN_THREADS - 1) / DEN_THREADS),
+                       dim3(DEN_THREADS), 0, c->stream, c->g, ro);
+    hipLaunchKernelGGL(k_den_partial, dim3((nk * f.npey + DEN_THREADS - 1) / DEN_THREADS),
+                       dim3(DEN_THREADS), 0, c->stream, c->g, (const double*)ro, c->den_buf,
+                       f.npey, (double)(f.npx * f.npy));
+    const size_t nkz = (size_t)nk;
+    double* rom = c->den_buf + nkz * f.npey;
+    hipLaunchKernelGGL(k_den_final, dim3((nk + DEN_THREADS - 1) / DEN_THREADS), dim3(DEN_THREADS),
+                       0, c->stream, (const double*)c->den_buf, rom, rom + nkz, rom + 2 * nkz,
+                       (const double*)(rom + 3 * nkz), nk, f.npey, f.id, f.dh * f.ore,
+                       f.dv * f.ore);
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
+}
+
 int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     const StageArgs a = stage_args(c, stage, zmode, want_dt);
-    const dim3 blk((c->ty + 3) * 64);
     hipError_t fma_err = hipSuccess;
-    int rc = timed(c, 0, [&] {
-#define R3D_STAGE(TY, S, Z, MG) \
-    hipLaunchKernelGGL((k_stage<TY, S, Z, MG>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
-#define R3D_STAGES(TY, MG)                                      \
-    if (stage == 0 && zmode == 0) R3D_STAGE(TY, 0, 0, MG);      \
-    else if (stage == 0) R3D_STAGE(TY, 0, 1, MG);               \
-    else if (stage == 1 && zmode == 0) R3D_STAGE(TY, 1, 0, MG); \
-    else if (stage == 1) R3D_STAGE(TY, 1, 2, MG);               \
-    else R3D_STAGE(TY, 2, 2, MG);
+    int rc;
+    if (c->cfg.id != 0 && (rc = launch_density_prep(c))) return rc;
+    rc = timed(c, 0, [&] {
+        const bool ht = c->cfg.itcon == 2, dn = c->cfg.id != 0;
         if (c->cfg.variant == 1) {
             fma_err = launch_stage_fma(a, c->C, c->M, c->ty, c->cfg.lmag != 0, c->grid,
                                        c->stream);
-        } else if (c->cfg.itcon == 2) {
-#define R3D_STAGE(TY, S, Z, MG) \
-    hipLaunchKernelGGL((k_stage<TY, S, Z, MG, true>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
-            if (c->cfg.lmag) {
-                R3D_STAGES(TILE_Y_MAG, true)
-            } else {
-                R3D_STAGES(8, false)
-            }
-#undef R3D_STAGE
-#define R3D_STAGE(TY, S, Z, MG) \
-    hipLaunchKernelGGL((k_stage<TY, S, Z, MG>), dim3(c->grid), blk, 0, c->stream, a, c->C, c->M)
         } else if (c->cfg.lmag) {
-            R3D_STAGES(TILE_Y_MAG, true)
+            if (ht && dn) launch_stage_kernel<TILE_Y_MAG, true, true, true>(c, a, stage, zmode);
+            else if (ht) launch_stage_kernel<TILE_Y_MAG, true, true, false>(c, a, stage, zmode);
+            else if (dn) launch_stage_kernel<TILE_Y_MAG, true, false, true>(c, a, stage, zmode);
+            else launch_stage_kernel<TILE_Y_MAG, true>(c, a, stage, zmode);
+        } else if (ht || dn) {  // these families are built with 8-row tiles
+            if (ht && dn) launch_stage_kernel<8, false, true, true>(c, a, stage, zmode);
+            else if (ht) launch_stage_kernel<8, false, true, false>(c, a, stage, zmode);
+            else launch_stage_kernel<8, false, false, true>(c, a, stage, zmode);
+        } else if (c->ty == 8) {
+            launch_stage_kernel<8, false>(c, a, stage, zmode);
         } else {
-            if (c->ty == 8) {
-                R3D_STAGES(8, false)
-            } else {
-                R3D_STAGES(TILE_Y, false)
-            }
+            launch_stage_kernel<TILE_Y, false>(c, a, stage, zmode);
         }
-#undef R3D_STAGES
-#undef R3D_STAGE
     });
     if (rc) return rc;
     if (fma_err != hipSuccess)
@@ -394,6 +477,10 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
         return set_err(R3D_EUNSUP, "FLUXES: unsupported feature enabled: ITCON=3 embedded thermal");
     if (f.itcon == 2 && f.variant != 0)
         return set_err(R3D_EUNSUP, "ITCON=2 heat loss is built in the bit-exact variant only");
+    if (f.id != 0 && f.id != 1 && f.id != 2)
+        return set_err(R3D_EUNSUP, "density diffusion ID=%d: 1 and 2 are defined", f.id);
+    if (f.id != 0 && f.variant != 0)
+        return set_err(R3D_EUNSUP, "density diffusion is built in the bit-exact variant only");
     if (f.variant != 0 && f.variant != 1)
         return set_err(R3D_EUNSUP, "variant %d: 0 (bit-exact) and 1 (FMA) are built", f.variant);
     HIPCHK(hipSetDevice(f.device));
@@ -415,7 +502,7 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     fill_consts(c);
 
     // stage tiling: 64 x ty columns, z chunks sized for >= ~8 blocks per CU
-    c->ty = f.lmag ? TILE_Y_MAG : f.itcon == 2 ? 8 : hydro_tile_rows(f.npy);
+    c->ty = f.lmag ? TILE_Y_MAG : (f.itcon == 2 || f.id != 0) ? 8 : hydro_tile_rows(f.npy);
     c->ntx = (f.npx + 63) / 64;
     c->nty = (f.npy + c->ty - 1) / c->ty;
     c->kc = 8;
@@ -460,6 +547,13 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     const size_t nmet = 3 * (size_t)(f.npx + 2) + 3 * (size_t)(f.npy + 2) + 5 * (size_t)(f.npz + 2) +
                         2 * ((size_t)(f.npx + 2) + (size_t)(f.npy + 2)) + (size_t)(f.npz + 2);
     if ((rc = dev_alloc(c, &c->metric_buf, nmet))) return fail(rc);
+    if (f.id != 0) {
+        const size_t nk = (size_t)f.npz + 2;
+        if ((rc = dev_alloc(c, &c->den_buf, nk * ((size_t)f.npey + 4)))) return fail(rc);
+        c->M.rom = c->den_buf + nk * f.npey;
+        c->M.dth = c->M.rom + nk;
+        c->M.dtv = c->M.dth + nk;
+    }
     const size_t stage_elems = (size_t)std::max<long>((long)c->nx * c->ny * c->nz,
                                                       (long)(f.npx + 2) * (f.npy + 2));
     if ((rc = dev_alloc(c, &c->stage_buf, stage_elems))) return fail(rc);
@@ -482,6 +576,7 @@ int r3d_hip_destroy(r3d_hip_ctx* c) {
     if (c->partial) (void)hipFree(c->partial);
     if (c->scal) (void)hipFree(c->scal);
     if (c->metric_buf) (void)hipFree(c->metric_buf);
+    if (c->den_buf) (void)hipFree(c->den_buf);
     if (c->stage_buf) (void)hipFree(c->stage_buf);
     if (c->gstage) (void)hipFree(c->gstage);
     if (c->copy_stream) {
@@ -556,6 +651,21 @@ int r3d_hip_set_metrics(r3d_hip_ctx* c, const r3d_hip_metrics* m) {
     M.hgz = put(hg.data() + X + Y, Z);
     M.dz_top = f.dzf / m->dzzdz[1];
     M.dz_bot = f.dzf / m->dzzdz[f.npz];
+    // ID=2 density diffusion profile WWZ = EXP(CLN*ZEE**2) + EXP(CLN*(ZEE-ZMAX)**2),
+    // SGM = 0.1E00 (REAL*8 from a single literal), CLN = -4.0E00*LOG(2.0E00)/SGM/SGM
+    // (golden/fortran/3dmhdsub.f:1334-1338)
+    std::vector<double> wwz2(Z, 0.0);
+    if (f.id == 2) {
+        if (!m->zee) return set_err(R3D_EINVAL, "metric array zee missing (ID=2)");
+        const double sgm = (double)0.1f;
+        const double wcln = (double)(-4.0f * logf(2.0f)) / sgm / sgm;
+        for (int k = 0; k < Z; ++k)
+            wwz2[k] = exp(wcln * (m->zee[k] * m->zee[k])) +
+                      exp(wcln * ((m->zee[k] - f.zmax) * (m->zee[k] - f.zmax)));
+    }
+    if (c->den_buf)
+        HIPCHK(hipMemcpyAsync(c->den_buf + (size_t)Z * (f.npey + 3), wwz2.data(), Z * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->metric_buf, h.data(), h.size() * sizeof(double),
                           hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

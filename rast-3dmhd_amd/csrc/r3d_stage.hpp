@@ -205,15 +205,17 @@ __device__ __forceinline__ void block_partials(double pm[NPART], double (*red)[N
 // One tile t (0 <= t < a.nblocks) of one RK stage; the caller maps blocks to
 // tiles.  ZMODE: 0 = Z' is the next stage's mix, 1 = Z keeps the saved state
 // (a step truncated after stage 1), 2 = Z is not written (last stage).
-// HEAT: the ITCON=2 embedded heat loss term is added to FT (a separate
-// instantiation, so that the default kernels carry none of its registers).
-template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false>
+// HEAT: the ITCON=2 embedded heat loss term is added to FT; DEN: density
+// diffusion is added to FR.  Separate instantiations, so that the default
+// kernels carry none of their registers or LDS.
+template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false, bool DEN = false>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, const Metrics& M,
                                            const int t) {
     constexpr int ROWS = TY + 2;
     constexpr int NW = TY + 3;
     constexpr int NF = MAG ? 8 : 5;  // primaries per state
-    __shared__ double sh[2][MAG ? NSH_MAG : NSH][ROWS][66];
+    constexpr int SH_RO = MAG ? NSH_MAG : NSH;  // DEN only: raw RO of the plane
+    __shared__ double sh[2][(MAG ? NSH_MAG : NSH) + (DEN ? 1 : 0)][ROWS][66];
     __shared__ double red[NW][NPART];
     // Per-plane z metrics of this block's chunk, staged once: they are
     // uniform per plane, and fetching them inside the z loop would put a
@@ -357,6 +359,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 sh[buf][SH_BY][r][c] = cc.by;
                 sh[buf][SH_BZ][r][c] = cc.bz;
             }
+            if constexpr (DEN) sh[buf][SH_RO][r][c] = cc.ro;
         }
         __syncthreads();
 
@@ -423,6 +426,26 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 double fr = (SH(SH_RU, r, c - 1) - SH(SH_RU, r, c + 1)) * hxd * dxx;
                 fr = fr - (SH(SH_RV, r + 1, c) - SH(SH_RV, r - 1, c)) * tmpy;
                 fr = fr - w1;
+                if constexpr (DEN) {
+                    // diffusion of density perturbations WW1 = RO - ROM(K),
+                    // golden/fortran/3dmhdsub.f:1316-1380 (operation order kept)
+                    const double rm = *(const gdouble*)(M.rom + k);
+                    const double w0 = cc.ro - rm;
+                    if (C.den_h) {
+                        const double wE = SH(SH_RO, r, c + 1) - rm, wW = SH(SH_RO, r, c - 1) - rm;
+                        const double wN = SH(SH_RO, r + 1, c) - rm, wS = SH(SH_RO, r - 1, c) - rm;
+                        const double ww2 =
+                            (wE - wW) * hxd * d2xx + (wE - 2.0 * w0 + wW) * C.h2x * dxx * dxx;
+                        const double ww3 = (wN - wS) * tmpy1 + (wN - 2.0 * w0 + wS) * tmpy2;
+                        fr = fr + (ww2 + ww3) * *(const gdouble*)(M.dth + k);
+                    }
+                    if (C.den_v) {
+                        const double wU = cp.ro - *(const gdouble*)(M.rom + k + 1);
+                        const double wD = cm.ro - *(const gdouble*)(M.rom + k - 1);
+                        const double ww2 = (wU - wD) * tmpz1 + (wU - 2.0 * w0 + wD) * tmpz2;
+                        fr = fr + ww2 * *(const gdouble*)(M.dtv + k);
+                    }
+                }
                 finish(QRO, fr, cc.ro);
             }
             
@@ -683,12 +706,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
 }
 
 // One RK stage of one run: block -> tile with the XCD-aware order.
-template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false>
+template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false, bool DEN = false>
 __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, const Consts C,
                                                          const Metrics M) {
     const int t = logical_block();
     if (t >= a.nblocks) return;
-    stage_tile<TY, STAGE, ZMODE, MAG, HEAT>(a, C, M, t);
+    stage_tile<TY, STAGE, ZMODE, MAG, HEAT, DEN>(a, C, M, t);
 }
 
 }  // namespace R3D_KNS

@@ -722,7 +722,12 @@ int validate(const Params& p) {
     if (p.lshr) return fail("unsupported feature enabled: LSHR");
     if (p.lrem) return fail("unsupported feature enabled: LREM");
     if (p.rlax > 0.0) return fail("unsupported feature enabled: flux relaxation (RLAX)");
-    if (p.id != 0) return fail("unsupported feature enabled: density diffusion ID");
+    // density diffusion (golden/fortran/3dmhdsub.f:1306-1380) is built for ID
+    // 1 and 2; its horizontal mean exists for NGRID=0 only (src/communicate.cpp:190)
+    if (p.id != 0 && p.id != 1 && p.id != 2)
+        return fail("density diffusion ID=%d: 1 and 2 are defined", p.id);
+    if (p.id != 0 && p.ngrid != 0)
+        return fail("density diffusion needs NGRID=0 (horizontal_mean: only NGRID=0 is supported)");
     // ITCON=2 (embedded heat loss, golden/fortran/3dmhdsub.f:1797-1822) is
     // built; ITCON=3 has only a banner line in the Fortran (3dmhd.f:645, 1179)
     if (p.itcon == 3) return fail("unsupported feature enabled: ITCON=3 embedded thermal");
@@ -822,6 +827,10 @@ int r3d_host_cfg(const r3d_host* h, r3d_hip_cfg* o) {
     o->tb = h->tb;
     o->zp = p.zp;
     o->qfh = p.qfh;
+    o->dh = p.dh;
+    o->dv = p.dv;
+    o->zmax = p.zmax;
+    o->id = p.id;
     return 0;
 }
 

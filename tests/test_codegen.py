@@ -43,9 +43,9 @@ def asm(tmp_path_factory):
     return out.read_text()
 
 
-def kernel_body(asm, ty, stage, zmode, mag=False, heat=False):
-    name = (f"_ZN3r3d7k_stageILi{ty}ELi{stage}ELi{zmode}ELb{int(mag)}ELb{int(heat)}EEEv"
-            "NS_9StageArgsENS_6ConstsENS_7MetricsE")
+def kernel_body(asm, ty, stage, zmode, mag=False, heat=False, den=False):
+    name = (f"_ZN3r3d7k_stageILi{ty}ELi{stage}ELi{zmode}ELb{int(mag)}ELb{int(heat)}"
+            f"ELb{int(den)}EEEvNS_9StageArgsENS_6ConstsENS_7MetricsE")
     i = asm.index(name + ":")
     return asm[i:asm.index(".Lfunc_end", i)].split("\n")
 
@@ -83,8 +83,11 @@ def test_hydro_stage_kernel_memory_path(asm, ty, stage, zmode):
 
 
 @pytest.mark.parametrize("stage,zmode", FULL_STEP)
-@pytest.mark.parametrize("mag,heat", [(True, False), (False, True), (True, True)],
-                         ids=["mhd", "hydro-heat", "mhd-heat"])
-def test_other_stage_kernels_have_no_flat_access(asm, stage, zmode, mag, heat):
-    lines = kernel_body(asm, 5 if mag else 8, stage, zmode, mag=mag, heat=heat)
+@pytest.mark.parametrize("mag,heat,den", [(True, False, False), (False, True, False),
+                                          (True, True, False), (False, False, True),
+                                          (True, False, True), (False, True, True)],
+                         ids=["mhd", "hydro-heat", "mhd-heat", "hydro-den", "mhd-den",
+                              "hydro-heat-den"])
+def test_other_stage_kernels_have_no_flat_access(asm, stage, zmode, mag, heat, den):
+    lines = kernel_body(asm, 5 if mag else 8, stage, zmode, mag=mag, heat=heat, den=den)
     assert not [l for l in lines if re.match(r"\s+flat_", l)], "FLAT memory instruction"
