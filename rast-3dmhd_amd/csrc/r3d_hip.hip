@@ -114,17 +114,33 @@ __global__ __launch_bounds__(DEN_THREADS) void k_den_mirror(const Geo g, double*
 // Horizontal mean, first half (src/communicate.cpp:193-199): thread
 // (K, q) sums y rank q's interior rows of global plane K, j outer and i
 // inner, in one sequential chain, and divides by npx*npy.  The loads of a
-// row run ahead of the additions, which keep their order.
-__global__ __launch_bounds__(DEN_THREADS) void k_den_partial(const Geo g, const double* ro,
-                                                             double* part, int npey, double cnt) {
-    const int t = blockIdx.x * DEN_THREADS + threadIdx.x;
+// row run ahead of the additions, which keep their order.  There are only
+// (npz+2)*npey chains, so blocks are one wave each and spread over every CU:
+// the speed is then the number of loads each lane keeps in flight.
+constexpr int DEN_PARTIAL_THREADS = 64;
+__global__ __launch_bounds__(DEN_PARTIAL_THREADS) void k_den_partial(const Geo g,
+                                                                     const double* ro,
+                                                                     double* part, int npey,
+                                                                     double cnt) {
+    const int t = blockIdx.x * DEN_PARTIAL_THREADS + threadIdx.x;
     if (t >= (g.npz + 2) * npey) return;
     const int K = t / npey, q = t % npey, nry = g.npy / npey;
-    constexpr int U = 16;
+    // 16-byte loads (rows start on 128-byte boundaries), 32 in flight
+    constexpr int U2 = 32, U = 16;
     double sum = 0.0;
     for (int j = q * nry + 1; j <= (q + 1) * nry; ++j) {
         const double* row = ro + g.at(1, j, K);
         int i = 0;
+        for (; i + 2 * U2 <= g.npx; i += 2 * U2) {
+            double2 v[U2];
+#pragma unroll
+            for (int u = 0; u < U2; ++u) v[u] = *(const double2*)(row + i + 2 * u);
+#pragma unroll
+            for (int u = 0; u < U2; ++u) {
+                sum += v[u].x;
+                sum += v[u].y;
+            }
+        }
         for (; i + U <= g.npx; i += U) {
             double v[U];
 #pragma unroll
@@ -319,8 +335,9 @@ int launch_density_prep(r3d_hip_ctx* c) {
     const int nk = f.npz + 2, cells = (f.npx + 2) * (f.npy + 2);
     hipLaunchKernelGGL(k_den_mirror, dim3((cells + DEN_THREADS - 1) / DEN_THREADS),
                        dim3(DEN_THREADS), 0, c->stream, c->g, ro);
-    hipLaunchKernelGGL(k_den_partial, dim3((nk * f.npey + DEN_THREADS - 1) / DEN_THREADS),
-                       dim3(DEN_THREADS), 0, c->stream, c->g, (const double*)ro, c->den_buf,
+    hipLaunchKernelGGL(k_den_partial,
+                       dim3((nk * f.npey + DEN_PARTIAL_THREADS - 1) / DEN_PARTIAL_THREADS),
+                       dim3(DEN_PARTIAL_THREADS), 0, c->stream, c->g, (const double*)ro, c->den_buf,
                        f.npey, (double)(f.npx * f.npy));
     const size_t nkz = (size_t)nk;
     double* rom = c->den_buf + nkz * f.npey;

@@ -222,6 +222,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // global-memory round trip on every plane.
     constexpr int KCMAX = 128;
     __shared__ double zm[5][KCMAX];  // dzz, d2zz, ddz, rkapa, dkapa
+    // DEN / HEAT per-plane values, staged with the metrics for the same
+    // reason: ROM at k-1..k+1 (row 0, offset by one plane), DH*ORE*WWZ,
+    // DV*ORE*WWZ, then the heat loss z Gaussian; and the heat loss x / y
+    // Gaussians of the tile's columns and rows, so that they hold no
+    // registers across the z loop
+    constexpr int ZX_DEN = 0, ZX_HEAT = DEN ? 3 : 0;
+    constexpr int NZX = (DEN ? 3 : 0) + (HEAT ? 1 : 0);
+    __shared__ double zx[NZX > 0 ? NZX : 1][KCMAX + 2];
+    __shared__ double hgx_s[HEAT ? 66 : 1], hgy_s[HEAT ? ROWS : 1];
     // Stage 3: the column's min(ddx, ddy) and the lane's running dt extrema
     // live in LDS rather than in registers across the z loop.  Held in
     // registers they are spilled, and a scratch reload after a plane's stores
@@ -288,6 +297,29 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
             zm[f][kk - k0] = src[kk];
         }
     }
+    if constexpr (DEN) {
+        for (int q = threadIdx.x; q < KCMAX + 2; q += blockDim.x) {
+            const int kk = k0 - 1 + q;  // planes k0-1 .. k1+1 (within 0 .. npz+1)
+            if (kk <= k1 + 1) zx[ZX_DEN][q] = M.rom[kk];
+            if (q < KCMAX && k0 + q <= k1) {
+                zx[ZX_DEN + 1][q] = M.dth[k0 + q];
+                zx[ZX_DEN + 2][q] = M.dtv[k0 + q];
+            }
+        }
+    }
+    if constexpr (HEAT) {
+        for (int q = threadIdx.x; q < KCMAX; q += blockDim.x)
+            if (k0 + q <= k1) zx[ZX_HEAT][q] = M.hgz[k0 + q];
+        for (int q = threadIdx.x; q < 66 + ROWS; q += blockDim.x) {
+            if (q < 66) {  // column q: global i0 - 1 + q, wrapped as the lanes read it
+                const int gi2 = i0 - 1 + q;
+                hgx_s[q] = M.hgx[gi2 == 0 ? g.npx : (gi2 > g.npx ? 1 : gi2)];
+            } else {
+                const int gj2 = j0 - 1 + (q - 66);
+                hgy_s[q - 66] = M.hgy[gj2 == 0 ? g.npy : (gj2 > g.npy ? 1 : gj2)];
+            }
+        }
+    }
     __syncthreads();
 
     const double cg = C.gam[STAGE] * dt;
@@ -296,7 +328,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // x/y metric factors of this column (src/fluxes.cpp:90, 250-254)
     const double dxx = M.dxx[li], d2xx = M.d2xx[li];
     const double dyy = M.dyy[lj], d2yy = M.d2yy[lj];
-    const double hgx = HEAT ? M.hgx[li] : 0.0, hgy = HEAT ? M.hgy[lj] : 0.0;
     if (STAGE == 2 && heavy) {
         ddxy_s[r][c] = dmin(M.ddx[li], M.ddy[lj]);
         pm_s[0][r - 1][ln] = 0.0;
@@ -429,7 +460,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 if constexpr (DEN) {
                     // diffusion of density perturbations WW1 = RO - ROM(K),
                     // golden/fortran/3dmhdsub.f:1316-1380 (operation order kept)
-                    const double rm = *(const gdouble*)(M.rom + k);
+                    const double rm = zx[ZX_DEN][kl + 1];
                     const double w0 = cc.ro - rm;
                     if (C.den_h) {
                         const double wE = SH(SH_RO, r, c + 1) - rm, wW = SH(SH_RO, r, c - 1) - rm;
@@ -437,13 +468,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                         const double ww2 =
                             (wE - wW) * hxd * d2xx + (wE - 2.0 * w0 + wW) * C.h2x * dxx * dxx;
                         const double ww3 = (wN - wS) * tmpy1 + (wN - 2.0 * w0 + wS) * tmpy2;
-                        fr = fr + (ww2 + ww3) * *(const gdouble*)(M.dth + k);
+                        fr = fr + (ww2 + ww3) * zx[ZX_DEN + 1][kl];
                     }
                     if (C.den_v) {
-                        const double wU = cp.ro - *(const gdouble*)(M.rom + k + 1);
-                        const double wD = cm.ro - *(const gdouble*)(M.rom + k - 1);
+                        const double wU = cp.ro - zx[ZX_DEN][kl + 2];
+                        const double wD = cm.ro - zx[ZX_DEN][kl];
                         const double ww2 = (wU - wD) * tmpz1 + (wU - 2.0 * w0 + wD) * tmpz2;
-                        fr = fr + ww2 * *(const gdouble*)(M.dtv + k);
+                        fr = fr + ww2 * zx[ZX_DEN + 2][kl];
                     }
                 }
                 finish(QRO, fr, cc.ro);
@@ -548,8 +579,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 if constexpr (HEAT) {
                     // localized embedded heat loss, golden/fortran/3dmhdsub.f:1799-1822
                     // (RO there already holds 1/RO, :1405), same operation order
-                    const double hgz = *(const gdouble*)(M.hgz + k);
-                    ft = ft - C.hloss * cc.ri * hgx / C.hh * hgy / C.hh * hgz / C.hh;
+                    ft = ft - C.hloss * cc.ri * hgx_s[c] / C.hh * hgy_s[r] / C.hh *
+                                  zx[ZX_HEAT][kl] / C.hh;
                 }
                 const double ux = (SH(SH_U, r, c + 1) - SH(SH_U, r, c - 1)) * hxd * dxx;
                 const double vx = (SH(SH_V, r, c + 1) - SH(SH_V, r, c - 1)) * hxd * dxx;
