@@ -95,6 +95,16 @@ def load_package():
     return mod
 
 
+# Physics branches beyond the reference's time step, timed on the same grid.
+PHYSICS = {
+    "plain": [],
+    "heat": ["--itcon", "2", "--tp", "1.5", "--zp", "20.0", "--hh", "4.0"],
+    "density": ["--ngrid", "0", "--id", "1", "--dh", "0.4", "--dv", "0.3"],
+}
+PHYSICS_NAMES = {"heat": "ITCON=2 embedded heat loss (TP 1.5, ZP 20, HH 4)",
+                 "density": "density diffusion ID=1 (DH 0.4, DV 0.3, uniform grid)"}
+
+
 def run_adjoint(a):
     """Adjoint:forward time ratio of the hand-written discrete adjoint."""
     pkg = load_package()
@@ -274,6 +284,9 @@ def main():
                          "against plain steps and a blocking download")
     ap.add_argument("--variant", type=int, default=0, choices=(0, 1),
                     help="0: bit-exact stage kernel (default); 1: FMA-contracted stage kernel")
+    ap.add_argument("--physics", default="plain", choices=sorted(PHYSICS),
+                    help="plain (default); heat: the ITCON=2 embedded heat loss; density: "
+                         "density diffusion ID=1 (uniform grid)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="before the run, time one device-to-device copy of this many GiB "
@@ -299,10 +312,12 @@ def main():
         return run_adjoint(a)
     if a.checkpoint:
         return run_checkpoint(a)
-    extra = MHD_FLAGS if a.mhd else []
+    extra = (MHD_FLAGS if a.mhd else []) + PHYSICS[a.physics]
     bpc = BYTES_PER_CELL_STAGE_MHD if a.mhd else BYTES_PER_CELL_STAGE
-    cpu = None if (a.no_cpu_baseline or a.mhd) else cpu_baseline(a.npx, a.npy, a.npey, a.npez,
-                                                                  CPU_CORES)
+    plain = a.physics == "plain"
+    # the reference CPU build refuses the two extra physics branches
+    cpu = None if (a.no_cpu_baseline or a.mhd or not plain) else cpu_baseline(
+        a.npx, a.npy, a.npey, a.npez, CPU_CORES)
 
     if a.copy_calib_gib > 0:
         import torch
@@ -353,8 +368,8 @@ def main():
         # WRITE_SIZE passes of this workload (null for any other workload)
         try:
             for t in json.load(open(tfile)):
-                if (t.get("npx") == a.npx and t.get("npy") == a.npy and t.get("npz") == a.npz
-                        and bool(t.get("mhd", False)) == a.mhd):
+                if (plain and t.get("npx") == a.npx and t.get("npy") == a.npy
+                        and t.get("npz") == a.npz and bool(t.get("mhd", False)) == a.mhd):
                     traffic = t.get("hbm_bytes_per_launch")
         except (OSError, ValueError, AttributeError):
             traffic = None
@@ -375,7 +390,8 @@ def main():
         "config": {"workload": f"{'rotating MHD' if a.mhd else 'hydro'} "
                                f"{a.npx}x{a.npy}x{a.npz}, virtual "
                                f"{a.npey}x{a.npez} subdomains on one GPU, "
-                               f"{'FMA-contracted' if a.variant else 'bit-exact'} variant",
+                               f"{'FMA-contracted' if a.variant else 'bit-exact'} variant"
+                               + ("" if plain else f", {PHYSICS_NAMES[a.physics]}"),
                    "global_cells": cells, "stages_per_step": 3,
                    "device_bytes": mem, "hbm_roofline_frac_of_value":
                        value * bpc / (HBM_PEAK_GBS * 1e9)},
