@@ -4,7 +4,8 @@
   flight, returns exactly what r3d_hip_download_rank returns at the
   snapshot's point (ghost cells, RK accumulators, scalars).
 * Dump (from a snapshot) at step 3, restart with --nstart 3, 4 more steps:
-  bit-identical to 7 uninterrupted steps (state and scalars), hydro and MHD.
+  bit-identical to 7 uninterrupted steps (state and scalars), hydro, MHD,
+  and hydro with density diffusion and the ramped ITCON=2 heat loss.
 * The HIP path restarted from the reference CPU build's own step-2 dump and
   stepped once reproduces the reference's step-3 dump hashes.
 """
@@ -66,9 +67,15 @@ def test_snapshot_is_the_download_at_its_point(args):
     sim.close()
 
 
-@pytest.mark.parametrize("args", [HYDRO, MAG], ids=["hydro", "mhd"])
+# density diffusion (the dump holds the inverted mirror planes of RO) and the
+# TIMT-ramped heat loss (its amplitude must resume from the dumped TIMT)
+FEATURES = HYDRO + ["--ngrid", "0", "--id", "1", "--dh", "0.4", "--dv", "0.3", "--itcon", "2",
+                    "--tp", "1.5", "--zp", "27", "--hh", "6", "--tc", "0.06", "--qfh", "0.03"]
+
+
+@pytest.mark.parametrize("args", [HYDRO, MAG, FEATURES], ids=["hydro", "mhd", "features"])
 def test_restart_continues_bit_for_bit(args, tmp_path):
-    tag = "mhd" if "--lmag" in args else "hydro"
+    tag = "mhd" if "--lmag" in args else ("features" if "--id" in args else "hydro")
     say(f"[gpu] dump at step 3, restart, 4 more steps ({tag})")
     run = pkg.Run(args)
     names = X_FIELDS + (["bx", "by", "bz"] if "--lmag" in args else [])
