@@ -494,6 +494,8 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
         return set_err(R3D_EUNSUP, "FLUXES: unsupported feature enabled: ITCON=3 embedded thermal");
     if (f.itcon == 2 && f.variant != 0)
         return set_err(R3D_EUNSUP, "ITCON=2 heat loss is built in the bit-exact variant only");
+    if (f.itcon == 2 && f.tc != 0.0 && f.qfh == 0.0)
+        return set_err(R3D_EINVAL, "ITCON=2 with tc != 0 needs qfh != 0 (the heat-loss ramp time)");
     if (f.id != 0 && f.id != 1 && f.id != 2)
         return set_err(R3D_EUNSUP, "density diffusion ID=%d: 1 and 2 are defined", f.id);
     if (f.id != 0 && f.variant != 0)

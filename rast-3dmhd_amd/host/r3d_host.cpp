@@ -731,6 +731,10 @@ int validate(const Params& p) {
     // ITCON=2 (embedded heat loss, golden/fortran/3dmhdsub.f:1797-1822) is
     // built; ITCON=3 has only a banner line in the Fortran (3dmhd.f:645, 1179)
     if (p.itcon == 3) return fail("unsupported feature enabled: ITCON=3 embedded thermal");
+    // the ramp divides by QFH (golden/fortran/3dmhdsub.f:1806): without it the
+    // loss is inf or NaN from the first step on
+    if (p.itcon == 2 && p.tc != 0.0 && p.qfh == 0.0)
+        return fail("ITCON=2 with TC != 0 needs QFH != 0 (the heat-loss ramp time)");
     if (p.ixcon != 0 || p.iycon != 0) return fail("unsupported feature enabled: non-periodic horizontal boundaries");
     if (p.lmag && p.ntube != 0) return fail("unsupported feature enabled: TUBE models");
     if (p.ampb != 0.0 && !p.lmag) return fail("AMPB magnetic layer requires LMAG");

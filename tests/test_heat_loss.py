@@ -125,5 +125,7 @@ def test_host_accepts_itcon_2_and_refuses_itcon_3():
     assert cfg.itcon == 2 and cfg.zp == 27.0 and cfg.qfh == 0.5
     with pytest.raises(pkg.HipError, match="ITCON=3"):
         pkg.Run(BASE + ["--itcon", "3"])
+    with pytest.raises(pkg.HipError, match="QFH"):
+        pkg.Run(BASE + HEAT + ["--tc", "0.5"])
     with pytest.raises(RuntimeError, match="ITCON=3"):
         oracle.Sim(BASE + ["--itcon", "3"]).step(1)
