@@ -32,7 +32,7 @@ BYTES_PER_CELL_STAGE = 400.0 / 3.0   # hydro: 10 passes x 5 fields x 8 B per ste
 BYTES_PER_CELL_STAGE_MHD = 640.0 / 3.0  # MHD: 10 passes x 8 fields x 8 B per step
 MHD_FLAGS = ["--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2",
              "--bzp", "10", "--lrot", "1", "--ry", "0.1", "--ang", "0"]
-CPU_CORES = 16                        # the GPU box's CPU share for one GPU
+CPU_CORES = 16                        # host cores available per GPU
 
 
 def log(msg):

@@ -32,8 +32,8 @@ MPIRUN = "/opt/conda/bin/mpirun"
 
 # name -> (extra args, npey, npez, steps at which dumps are hashed)
 CONFIGS = {
-    # BASELINE config 0 geometry (hydro, 64x64x128, 1x2), 100 steps
-    "cfg0": (["--npx", "64", "--npy", "64", "--npz", "128"], 1, 2, [1, 2, 100]),
+    # the CPU-sized headline case (hydro, 64x64x128, 1x2), 100 steps
+    "h64_1x2": (["--npx", "64", "--npy", "64", "--npz", "128"], 1, 2, [1, 2, 100]),
     # the reference's own golden geometry (16x16x32 on 2x2)
     "g16_2x2": (["--npx", "16", "--npy", "16", "--npz", "32"], 2, 2, [1, 2, 3]),
     # decomposition family on one geometry

@@ -92,11 +92,11 @@ def test_hip_matches_reference_fixtures(golden, name):
     sim.close()
 
 
-def test_hip_config0_100_steps(golden):
-    """BASELINE config 0 geometry (64x64x128 on 1x2), 100 steps vs the reference."""
-    cfg = golden["cfg0"]
+def test_hip_h64_1x2_100_steps(golden):
+    """The CPU-sized headline case (hydro 64x64x128 on 1x2), 100 steps vs the reference."""
+    cfg = golden["h64_1x2"]
     args = cfg["args"] + ["--npey", "1", "--npez", "2"]
-    say("[gpu] cfg0 100 steps")
+    say("[gpu] h64_1x2 100 steps")
     run = pkg.Run(args)
     sim = run.make_sim()
     done = 0
@@ -104,7 +104,7 @@ def test_hip_config0_100_steps(golden):
         sim.step(st - done)
         done = st
         bad = check_golden(sim, cfg["dumps"][str(st)], 1)
-        assert not bad, f"cfg0 step {st}: {bad[:8]}"
+        assert not bad, f"h64_1x2 step {st}: {bad[:8]}"
 
 
 @pytest.mark.parametrize("nstages", [1, 2, 3])

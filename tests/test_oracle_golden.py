@@ -43,7 +43,7 @@ def compare(sim, want, npey):
 
 CASES = ["g16_2x2", "d32_1x2", "d32_2x2", "d32_4x2", "d32_2x4", "d32_1x4",
          "b_itcon0", "b_tp0", "b_itcon0_tp0", "b_itcon4", "b_izcon1", "b_rot", "b_tc",
-         "b_ngrid0", "b_ngrid2", "b_pzp", "b_re_theta", "u40x24_2x2", "cfg0"]
+         "b_ngrid0", "b_ngrid2", "b_pzp", "b_re_theta", "u40x24_2x2", "h64_1x2"]
 
 
 @pytest.mark.parametrize("name", CASES)
