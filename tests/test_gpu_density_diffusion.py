@@ -95,3 +95,14 @@ def test_density_diffusion_refused_where_not_built():
     cfg.variant = 1
     with pytest.raises(pkg.HipError, match="bit-exact variant only"):
         pkg.HipSim(cfg)
+
+
+@pytest.mark.parametrize("extra", [DEN, ["--itcon", "2", "--tp", "1.5", "--zp", "27", "--hh", "6"]],
+                         ids=["density", "heat"])
+def test_extra_physics_refused_by_ensemble_and_adjoint(extra):
+    args = ["--npx", "24", "--npy", "16", "--npz", "24", "--npey", "1", "--npez", "2"] + extra
+    run = pkg.Run(args)
+    with pytest.raises(pkg.HipError, match="not built in an ensemble"):
+        pkg.Ensemble([run, pkg.Run(args)])
+    with pytest.raises(pkg.HipError, match="adjoint"):
+        pkg.Adjoint(run.cfg(), run.metrics())
