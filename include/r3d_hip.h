@@ -68,6 +68,9 @@ typedef struct r3d_hip_cfg {
     double dh, dv, zmax;        /* density diffusion coefficients and domain depth */
     int id;                     /* density diffusion: 0 off, 1 WWZ = 1/ROM, 2 wall Gaussians
                                    (golden/fortran/3dmhdsub.f:1306-1380; needs NGRID=0) */
+    double rlax, tstart, toff;  /* flux relaxation (golden/fortran/3dmhdsub.f:1625-1771):
+                                   rlax > 0 turns it on; ITCON=0 with IZCON=1, NGRID=0,
+                                   npez >= 2 and toff > 0 only */
 } r3d_hip_cfg;
 
 /* Global 1-D metrics.  x arrays have npx+2 entries, y arrays npy+2, z arrays

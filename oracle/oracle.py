@@ -49,6 +49,12 @@ def lib():
         L.o_hmean.restype = ctypes.c_int
         L.o_rom.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.o_rom.restype = ctypes.POINTER(ctypes.c_double)
+        L.o_relax_prep.argtypes = [ctypes.c_void_p]
+        L.o_relax_prep.restype = ctypes.c_int
+        L.o_get_relax.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.o_get_relax.restype = None
+        L.o_correct.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.o_correct.restype = ctypes.POINTER(ctypes.c_double)
         L.o_communicate.argtypes = [ctypes.c_void_p]
         L.o_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.o_field.restype = ctypes.POINTER(ctypes.c_double)
@@ -129,6 +135,17 @@ class Sim:
 
     def rom(self, rank):
         return np.ctypeslib.as_array(lib().o_rom(self.h, rank), shape=(self.nz,))
+
+    def relax_prep(self):
+        self._check(lib().o_relax_prep(self.h))
+
+    def relax_state(self):
+        v = (ctypes.c_double * 2)()
+        lib().o_get_relax(self.h, v)
+        return {"tfac": v[0], "ttm_max": v[1]}
+
+    def correct(self, rank):
+        return np.ctypeslib.as_array(lib().o_correct(self.h, rank), shape=(self.nz,))
 
     def communicate(self):
         lib().o_communicate(self.h)

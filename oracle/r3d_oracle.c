@@ -450,6 +450,7 @@ typedef struct {
     double* f[NF3];
     double* m[NM1];
     double* rom; /* horizontal mean of RO per local plane (density diffusion) */
+    double *ttm, *fconm, *correct; /* flux relaxation, per local plane */
 } o_rank;
 
 struct o_sim {
@@ -458,6 +459,7 @@ struct o_sim {
     int nranks;
     o_rank* r;
     double dt, timc, timt, timi, umach, tb;
+    double tfac, ttm_max; /* flux relaxation state kept across calls (SAVE / DATA) */
     int nit;
 };
 
@@ -495,8 +497,13 @@ o_sim* o_create(int argc, const char* const* argv) {
                               s->d.ny, s->d.nz, s->d.nz, s->d.nz, s->d.nz, s->d.nz, s->d.nz};
         for (int m = 0; m < NM1; ++m) R->m[m] = calloc((size_t)len[m], sizeof(double));
         R->rom = calloc((size_t)s->d.nz, sizeof(double));
+        R->ttm = calloc((size_t)s->d.nz, sizeof(double));
+        R->fconm = calloc((size_t)s->d.nz, sizeof(double));
+        R->correct = calloc((size_t)s->d.nz, sizeof(double));
     }
     s->tb = 1.0;
+    s->tfac = 1.0;      /* DATA TFAC/1D0/, golden/fortran/3dmhdsub.f:1259 */
+    s->ttm_max = -1e99; /* DATA TTM_MAX/-1D99/, :1258 */
     return s;
 }
 
@@ -506,6 +513,9 @@ void o_free(o_sim* s) {
         for (int f = 0; f < NF3; ++f) free(s->r[r].f[f]);
         for (int m = 0; m < NM1; ++m) free(s->r[r].m[m]);
         free(s->r[r].rom);
+        free(s->r[r].ttm);
+        free(s->r[r].fconm);
+        free(s->r[r].correct);
     }
     free(s->r);
     free(s);
@@ -519,6 +529,14 @@ void o_dims(const o_sim* s, int dims[10]) {
 double* o_field(o_sim* s, int rank, int fid) {
     if (rank < 0 || rank >= s->nranks || fid < 0 || fid >= NF3) return NULL;
     return s->r[rank].f[fid];
+}
+void o_get_relax(const o_sim* s, double v[2]) {
+    v[0] = s->tfac;
+    v[1] = s->ttm_max;
+}
+double* o_correct(o_sim* s, int rank) {
+    if (rank < 0 || rank >= s->nranks) return NULL;
+    return s->r[rank].correct;
 }
 double* o_rom(o_sim* s, int rank) {
     if (rank < 0 || rank >= s->nranks) return NULL;

@@ -41,6 +41,12 @@
  *                            by discrete-Laplacian closed forms (cosine modes,
  *                            quadratic z profiles, the mirrored wall planes,
  *                            1/ROM and the ID=2 wall profile).
+ *   flux relaxation RLAX ... parity unpinned: the reference C++ refuses RLAX
+ *                            (src/fluxes.cpp:48); restated from
+ *                            golden/fortran/3dmhdsub.f:1534-1547, 1625-1771 for
+ *                            ITC=0 / IZC=1 and checked by closed forms (linear
+ *                            T: a linear correction across z ranks; FCONM of
+ *                            cosine and uniform flows; TFAC steps and cap).
  */
 #ifndef R3D_ORACLE_H
 #define R3D_ORACLE_H
@@ -75,6 +81,11 @@ int o_density_prep(o_sim* s);
 /* Horizontal mean of field fid into every rank's o_rom() (src/communicate.cpp:184). */
 int o_hmean(o_sim* s, int fid);
 double* o_rom(o_sim* s, int rank);
+/* Flux relaxation before FLUXES (RLAX > 0): TTM, TFAC, FCONM, CORRECT
+ * (golden/fortran/3dmhdsub.f:1534-1547, 1625-1771).  o_step calls it. */
+int o_relax_prep(o_sim* s);
+void o_get_relax(const o_sim* s, double v[2]); /* TFAC, TTM_MAX */
+double* o_correct(o_sim* s, int rank);         /* CORRECT per local plane */
 void o_communicate(o_sim* s);          /* src/communicate.cpp:171 */
 
 /* Introspection.  dims = {nx, ny, nz, npey, npez, nranks, npx, npy, npz, lmag}. */

@@ -43,7 +43,8 @@ class Cfg(ctypes.Structure):
                  "re", "repr", "ore", "cv", "ocv", "gamma", "grav", "theta", "sff",
                  "rm", "orm", "obeta", "omx", "omz",
                  "tp", "tc", "hh", "xp", "yp", "dzf", "tb", "zp", "qfh", "dh", "dv", "zmax")] + \
-               [("id", ctypes.c_int)]
+               [("id", ctypes.c_int)] + \
+               [(n, ctypes.c_double) for n in ("rlax", "tstart", "toff")]
 
 
 METRIC_NAMES = ["exx", "dxxdx", "d2xxdx2", "ddx", "wyy", "dyydy", "d2yydy2", "ddy",

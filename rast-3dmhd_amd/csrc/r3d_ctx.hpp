@@ -52,8 +52,10 @@ struct r3d_hip_ctx {
     double* partial = nullptr;
     r3d::DevScal* scal = nullptr;
     double* metric_buf = nullptr;
-    // density diffusion: per-(plane, y rank) partial means, then ROM, the two
-    // coefficient rows and the ID=2 wall profile, (npz+2) each
+    // per-plane extras (density diffusion, flux relaxation): per-(plane, y
+    // rank) partial means, then ROM, the two diffusion coefficient rows, the
+    // ID=2 wall profile, TTM, FCONM and RLAX*TFAC*CORRECT, (npz+2) each, and
+    // the relaxation state TFAC, TTM_MAX
     double* den_buf = nullptr;
     double* stage_buf = nullptr;
     double* gstage = nullptr;  // one global field, (npx+2)(npy+2)(npz+2), allocated on first use

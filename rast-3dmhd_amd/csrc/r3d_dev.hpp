@@ -61,6 +61,7 @@ struct Consts {
     // density diffusion (ID != 0): the horizontal and vertical terms are on
     // (DH != 0, DV != 0); read by the DEN stage kernels only
     int den_h, den_v;
+    int rlx;  // flux relaxation on: FT gains the per-plane RLAX*TFAC*CORRECT (DEN kernels)
     int lrot, lmag, top_mode, half_t, izcon, bz_zero;
     int mag_bot;  // LMAG one-sided z stencils on the bottom wall (npez > 1,
                   // golden/fortran/3dmhdsub.f:1126, 1392, 1513 sit in an ELSE IF)
@@ -77,6 +78,9 @@ struct Metrics {
     // density diffusion per global plane (0..npz+1), rewritten before every
     // stage: ROM (horizontal mean of RO), DH*ORE*WWZ and DV*ORE*WWZ
     const double *rom, *dth, *dtv;
+    // flux relaxation per global plane: RLAX*TFAC*CORRECT(K), rewritten
+    // before every stage; and the plane coordinates
+    const double *ftc, *zee;
     double dz_top, dz_bot;  // f4(1/(npz-1)) / dzzdz at the boundary planes
 };
 

@@ -263,6 +263,8 @@ int r3d_ens_create(int nmembers, const r3d_hip_cfg* cfgs, r3d_ens_ctx** out) {
             return set_err(R3D_EUNSUP, "member %d: tc != 0 is not supported in an ensemble", m);
         if (cfgs[m].id != 0)
             return set_err(R3D_EUNSUP, "member %d: density diffusion is not built in an ensemble", m);
+        if (cfgs[m].rlax > 0.0)
+            return set_err(R3D_EUNSUP, "member %d: flux relaxation is not built in an ensemble", m);
         if (cfgs[m].itcon == 2)
             return set_err(R3D_EUNSUP, "member %d: ITCON=2 heat loss is not built in an ensemble", m);
     }

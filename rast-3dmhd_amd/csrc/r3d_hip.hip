@@ -118,10 +118,8 @@ __global__ __launch_bounds__(DEN_THREADS) void k_den_mirror(const Geo g, double*
 // (npz+2)*npey chains, so blocks are one wave each and spread over every CU:
 // the speed is then the number of loads each lane keeps in flight.
 constexpr int DEN_PARTIAL_THREADS = 64;
-__global__ __launch_bounds__(DEN_PARTIAL_THREADS) void k_den_partial(const Geo g,
-                                                                     const double* ro,
-                                                                     double* part, int npey,
-                                                                     double cnt) {
+__device__ __forceinline__ void hmean_partial_chain(const Geo& g, const double* ro, double* part,
+                                                    int npey, double cnt) {
     const int t = blockIdx.x * DEN_PARTIAL_THREADS + threadIdx.x;
     if (t >= (g.npz + 2) * npey) return;
     const int K = t / npey, q = t % npey, nry = g.npy / npey;
@@ -152,6 +150,105 @@ __global__ __launch_bounds__(DEN_PARTIAL_THREADS) void k_den_partial(const Geo g
     }
     part[t] = sum / cnt;
 }
+__global__ __launch_bounds__(DEN_PARTIAL_THREADS) void k_den_partial(const Geo g,
+                                                                     const double* ro,
+                                                                     double* part, int npey,
+                                                                     double cnt) {
+    hmean_partial_chain(g, ro, part, npey, cnt);
+}
+
+// The convective flux WW2 = -RW*(2.5*(TT-TTM(K)) + 0.5*(UU*UU+VV*VV+WW*WW)),
+// UU = RU*(1/RO) (golden/fortran/3dmhdsub.f:1656-1666), summed like
+// k_den_partial: one sequential chain per (plane, y rank), rows j outer.
+__global__ __launch_bounds__(DEN_PARTIAL_THREADS) void k_ww2_partial(const Geo g, const State x,
+                                                                     const double* ttm,
+                                                                     double* part, int npey,
+                                                                     double cnt) {
+    const int t = blockIdx.x * DEN_PARTIAL_THREADS + threadIdx.x;
+    if (t >= (g.npz + 2) * npey) return;
+    const int K = t / npey, q = t % npey, nry = g.npy / npey;
+    const double tm = ttm[K];
+    double sum = 0.0;
+    for (int j = q * nry + 1; j <= (q + 1) * nry; ++j) {
+        const long n0 = g.at(1, j, K);
+        for (int i = 0; i < g.npx; ++i) {
+            const long n = n0 + i;
+            const double rw = x.q[QRW][n], ri = 1.0 / x.q[QRO][n];
+            const double uu = x.q[QRU][n] * ri, vv = x.q[QRV][n] * ri, ww = rw * ri;
+            sum += -rw * (2.5 * (x.q[QTT][n] - tm) + 0.5 * (uu * uu + vv * vv + ww * ww));
+        }
+    }
+    part[t] = sum / cnt;
+}
+
+// The y ranks' partial means of one plane added in rank order.
+__global__ __launch_bounds__(DEN_THREADS) void k_hmean_final(const double* part, double* out,
+                                                             int nk, int npey) {
+    const int K = blockIdx.x * DEN_THREADS + threadIdx.x;
+    if (K >= nk) return;
+    double acc = part[(long)K * npey];
+    for (int q = 1; q < npey; ++q) acc += part[(long)K * npey + q];
+    out[K] = acc;
+}
+
+// Flux relaxation, one thread (golden/fortran/3dmhdsub.f:1625-1771, the
+// ITC=0 / IZC=1 branch): the TFAC update from the bottom-wall TTM, then per z
+// rank (nrz planes, the rank's lower ghost plane being the next rank's first)
+// FRADM, ALPHA and the trapezoid integral CORRECT, offset by the sum of the
+// ranks above's end values, and ftc = RLAX*TFAC*CORRECT for the interior.
+__global__ void k_rlax_chain(const DevScal* sc, int stage, double* st, const double* ttm,
+                             const double* fconm, double* ftc, const Metrics M, int npz, int npez,
+                             double hz, double repr, double theta, double rlax, double tstart,
+                             double toff) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double dt = stage == 0 ? sc->dt_next : sc->dt;  // the step's dt
+    const double timt = sc->timc + sc->timi;
+    double tfac = st[0];
+    const bool on = timt > tstart && tfac > 1e-4;
+    if (on) {
+        const double tbot = ttm[npz];
+        if (tbot <= st[1]) {
+            tfac = tfac - dt / (3.0 * toff);
+        } else {
+            st[1] = tbot;
+            tfac = tfac + dt / (3.0 * toff);
+            tfac = fmin(tfac, 2.0);
+        }
+        st[0] = tfac;
+    }
+    const double rt = rlax * tfac;
+    if (!on) {
+        for (int G = 1; G <= npz; ++G) ftc[G] = rt * 0.0;
+        return;
+    }
+    const int nrz = npz / npez;
+    const double ftot = theta / repr;
+    double sum = 0.0;
+    for (int pz = 0; pz < npez; ++pz) {
+        double corr = 0.0, aprev = 0.0;
+        for (int k = 1; k <= nrz + 1; ++k) {
+            const int G = pz * nrz + k;
+            double fr;
+            if (k == nrz + 1 && pz == npez - 1) {
+                fr = 0.0;  // never assigned in the Fortran; only in this rank's unused end value
+            } else if (pz == 0 && k == 1) {
+                fr = (-3.0 * ttm[1] + 4.0 * ttm[2] - ttm[3]) * hz * M.dzz[1] * M.rkapa[1] / repr;
+            } else if (pz == npez - 1 && k == nrz) {
+                fr = (3.0 * ttm[npz] - 4.0 * ttm[npz - 1] + ttm[npz - 2]) * hz * M.dzz[npz] *
+                     M.rkapa[npz] / repr;
+            } else {
+                fr = (ttm[G + 1] - ttm[G - 1]) * hz * M.dzz[G] * M.rkapa[G] / repr;
+            }
+            double a = 1.0 - (fconm[G] + fr) / ftot;
+            a = a / (1.0 + 4.0 * fabs(a));
+            corr = k == 1 ? 0.0 : corr + 0.5 * (a + aprev) * (M.zee[G] - M.zee[G - 1]);
+            aprev = a;
+            if (k <= nrz) ftc[G] = rt * (corr + sum);
+        }
+        sum = sum + corr;  // this rank's end value, CORRECT(NZ)
+    }
+}
+
 
 // Second half (src/communicate.cpp:205-211): the y ranks' partial means
 // added in rank order; then the per-plane coefficients TMP = DH*ORE*WWZ(K)
@@ -233,6 +330,7 @@ void fill_consts(r3d_hip_ctx* c) {
     C.hloss = heat_amplitude(f, 0.0);
     C.den_h = f.id != 0 && f.dh != 0.0;
     C.den_v = f.id != 0 && f.dv != 0.0;
+    C.rlx = f.rlax > 0.0;
     C.gam[0] = f.gam1;
     C.gam[1] = f.gam2;
     C.gam[2] = f.gam3;
@@ -349,13 +447,46 @@ int launch_density_prep(r3d_hip_ctx* c) {
     return R3D_OK;
 }
 
+// Flux relaxation set-up of this stage's input: TTM, FCONM, then TFAC and
+// RLAX*TFAC*CORRECT per plane (after the density set-up, whose mirrored RO
+// planes the lower ghost plane's convective flux reads, as in the Fortran).
+int launch_relax_prep(r3d_hip_ctx* c, int stage) {
+    const r3d_hip_cfg& f = c->cfg;
+    const int nk = f.npz + 2;
+    const size_t nkz = (size_t)nk;
+    double* part = c->den_buf;
+    double* ttm = c->den_buf + nkz * (f.npey + 4);
+    double* fconm = ttm + nkz;
+    double* ftc = fconm + nkz;
+    double* st = c->den_buf + nkz * (f.npey + 7);
+    State s;
+    for (int q = 0; q < NXF; ++q) s.q[q] = c->xs[c->cur][q];
+    const double cnt = (double)(f.npx * f.npy);
+    const dim3 gp((nk * f.npey + DEN_PARTIAL_THREADS - 1) / DEN_PARTIAL_THREADS);
+    const dim3 gf((nk + DEN_THREADS - 1) / DEN_THREADS);
+    hipLaunchKernelGGL(k_den_partial, gp, dim3(DEN_PARTIAL_THREADS), 0, c->stream, c->g,
+                       (const double*)s.q[QTT], part, f.npey, cnt);
+    hipLaunchKernelGGL(k_hmean_final, gf, dim3(DEN_THREADS), 0, c->stream, (const double*)part, ttm,
+                       nk, f.npey);
+    hipLaunchKernelGGL(k_ww2_partial, gp, dim3(DEN_PARTIAL_THREADS), 0, c->stream, c->g, s,
+                       (const double*)ttm, part, f.npey, cnt);
+    hipLaunchKernelGGL(k_hmean_final, gf, dim3(DEN_THREADS), 0, c->stream, (const double*)part,
+                       fconm, nk, f.npey);
+    hipLaunchKernelGGL(k_rlax_chain, dim3(1), dim3(64), 0, c->stream, (const DevScal*)c->scal, stage,
+                       st, (const double*)ttm, (const double*)fconm, ftc, c->M, f.npz, f.npez, f.hz,
+                       f.repr, f.theta, f.rlax, f.tstart, f.toff);
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
+}
+
 int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     const StageArgs a = stage_args(c, stage, zmode, want_dt);
     hipError_t fma_err = hipSuccess;
     int rc;
     if (c->cfg.id != 0 && (rc = launch_density_prep(c))) return rc;
+    if (c->cfg.rlax > 0.0 && (rc = launch_relax_prep(c, stage))) return rc;
     rc = timed(c, 0, [&] {
-        const bool ht = c->cfg.itcon == 2, dn = c->cfg.id != 0;
+        const bool ht = c->cfg.itcon == 2, dn = c->cfg.id != 0 || c->cfg.rlax > 0.0;
         if (c->cfg.variant == 1) {
             fma_err = launch_stage_fma(a, c->C, c->M, c->ty, c->cfg.lmag != 0, c->grid,
                                        c->stream);
@@ -500,6 +631,14 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
         return set_err(R3D_EUNSUP, "density diffusion ID=%d: 1 and 2 are defined", f.id);
     if (f.id != 0 && f.variant != 0)
         return set_err(R3D_EUNSUP, "density diffusion is built in the bit-exact variant only");
+    if (f.rlax > 0.0) {
+        if (f.variant != 0)
+            return set_err(R3D_EUNSUP, "flux relaxation is built in the bit-exact variant only");
+        if (!(f.itcon == 0 && f.izcon == 1))
+            return set_err(R3D_EUNSUP, "flux relaxation (RLAX) is defined for ITCON=0 with IZCON=1 only");
+        if (f.npez < 2 || !(f.toff > 0.0))
+            return set_err(R3D_EINVAL, "flux relaxation needs npez >= 2 and toff > 0");
+    }
     if (f.variant != 0 && f.variant != 1)
         return set_err(R3D_EUNSUP, "variant %d: 0 (bit-exact) and 1 (FMA) are built", f.variant);
     HIPCHK(hipSetDevice(f.device));
@@ -521,7 +660,8 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     fill_consts(c);
 
     // stage tiling: 64 x ty columns, z chunks sized for >= ~8 blocks per CU
-    c->ty = f.lmag ? TILE_Y_MAG : (f.itcon == 2 || f.id != 0) ? 8 : hydro_tile_rows(f.npy);
+    c->ty = f.lmag ? TILE_Y_MAG
+                   : (f.itcon == 2 || f.id != 0 || f.rlax > 0.0) ? 8 : hydro_tile_rows(f.npy);
     c->ntx = (f.npx + 63) / 64;
     c->nty = (f.npy + c->ty - 1) / c->ty;
     c->kc = 8;
@@ -564,14 +704,23 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
             return fail(set_err(R3D_EHIP, "hipMemcpy scalars failed"));
     }
     const size_t nmet = 3 * (size_t)(f.npx + 2) + 3 * (size_t)(f.npy + 2) + 5 * (size_t)(f.npz + 2) +
-                        2 * ((size_t)(f.npx + 2) + (size_t)(f.npy + 2)) + (size_t)(f.npz + 2);
+                        2 * ((size_t)(f.npx + 2) + (size_t)(f.npy + 2)) + 2 * (size_t)(f.npz + 2);
     if ((rc = dev_alloc(c, &c->metric_buf, nmet))) return fail(rc);
-    if (f.id != 0) {
-        const size_t nk = (size_t)f.npz + 2;
-        if ((rc = dev_alloc(c, &c->den_buf, nk * ((size_t)f.npey + 4)))) return fail(rc);
+    if (f.id != 0 || f.rlax > 0.0) {
+        const size_t nk = (size_t)f.npz + 2, n = nk * ((size_t)f.npey + 7) + 2;
+        if ((rc = dev_alloc(c, &c->den_buf, n))) return fail(rc);
+        // zero rows the run does not compute; TFAC = 1, TTM_MAX = -1e99
+        // (DATA statements, golden/fortran/3dmhdsub.f:1258-1259)
+        std::vector<double> init(n, 0.0);
+        init[n - 2] = 1.0;
+        init[n - 1] = -1e99;
+        if (hipMemcpy(c->den_buf, init.data(), n * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return fail(set_err(R3D_EHIP, "hipMemcpy of the per-plane buffer failed"));
         c->M.rom = c->den_buf + nk * f.npey;
         c->M.dth = c->M.rom + nk;
         c->M.dtv = c->M.dth + nk;
+        c->M.ftc = c->M.rom + 6 * nk;
     }
     const size_t stage_elems = (size_t)std::max<long>((long)c->nx * c->ny * c->nz,
                                                       (long)(f.npx + 2) * (f.npy + 2));
@@ -645,7 +794,10 @@ int r3d_hip_set_metrics(r3d_hip_ctx* c, const r3d_hip_metrics* m) {
             hg[X + Y + k] = exp(hcln * ((m->zee[k] - f.zp) * (m->zee[k] - f.zp)));
     }
     std::vector<double> h;
-    h.reserve(3 * X + 3 * Y + 5 * Z + 2 * (X + Y) + Z);
+    h.reserve(3 * X + 3 * Y + 5 * Z + 2 * (X + Y) + 2 * Z);
+    if (f.rlax > 0.0 && !m->zee) return set_err(R3D_EINVAL, "metric array zee missing (RLAX)");
+    std::vector<double> zee(Z, 0.0);
+    if (m->zee) zee.assign(m->zee, m->zee + Z);
     auto put = [&](const double* p, int n) -> const double* {
         const size_t off = h.size();
         h.insert(h.end(), p, p + n);
@@ -668,6 +820,7 @@ int r3d_hip_set_metrics(r3d_hip_ctx* c, const r3d_hip_metrics* m) {
     M.hgx = put(hg.data(), X);
     M.hgy = put(hg.data() + X, Y);
     M.hgz = put(hg.data() + X + Y, Z);
+    M.zee = put(zee.data(), Z);
     M.dz_top = f.dzf / m->dzzdz[1];
     M.dz_bot = f.dzf / m->dzzdz[f.npz];
     // ID=2 density diffusion profile WWZ = EXP(CLN*ZEE**2) + EXP(CLN*(ZEE-ZMAX)**2),

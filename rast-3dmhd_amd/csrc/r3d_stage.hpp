@@ -224,11 +224,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     __shared__ double zm[5][KCMAX];  // dzz, d2zz, ddz, rkapa, dkapa
     // DEN / HEAT per-plane values, staged with the metrics for the same
     // reason: ROM at k-1..k+1 (row 0, offset by one plane), DH*ORE*WWZ,
-    // DV*ORE*WWZ, then the heat loss z Gaussian; and the heat loss x / y
+    // DV*ORE*WWZ, RLAX*TFAC*CORRECT, then the heat loss z Gaussian; and the heat loss x / y
     // Gaussians of the tile's columns and rows, so that they hold no
     // registers across the z loop
-    constexpr int ZX_DEN = 0, ZX_HEAT = DEN ? 3 : 0;
-    constexpr int NZX = (DEN ? 3 : 0) + (HEAT ? 1 : 0);
+    constexpr int ZX_DEN = 0, ZX_HEAT = DEN ? 4 : 0;
+    constexpr int NZX = (DEN ? 4 : 0) + (HEAT ? 1 : 0);
     __shared__ double zx[NZX > 0 ? NZX : 1][KCMAX + 2];
     __shared__ double hgx_s[HEAT ? 66 : 1], hgy_s[HEAT ? ROWS : 1];
     // Stage 3: the column's min(ddx, ddy) and the lane's running dt extrema
@@ -304,6 +304,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
             if (q < KCMAX && k0 + q <= k1) {
                 zx[ZX_DEN + 1][q] = M.dth[k0 + q];
                 zx[ZX_DEN + 2][q] = M.dtv[k0 + q];
+                zx[ZX_DEN + 3][q] = M.ftc[k0 + q];
             }
         }
     }
@@ -576,6 +577,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 ft = ft + ((ttN - ttS) * tmpy1 + (ttN - t2c + ttS) * tmpy2) * cc.ri * C.tk * rk;
                 ft = ft + ((ttU - ttD) * tmpz1 + (ttU - t2c + ttD) * tmpz2) * cc.ri * C.tk * rk;
                 ft = ft + (ttU - ttD) * C.hz * dzz * cc.ri * C.tk * dk;
+                if constexpr (DEN) {  // flux relaxation, golden/fortran/3dmhdsub.f:1765-1771
+                    if (C.rlx) ft = ft + zx[ZX_DEN + 3][kl];
+                }
                 if constexpr (HEAT) {
                     // localized embedded heat loss, golden/fortran/3dmhdsub.f:1799-1822
                     // (RO there already holds 1/RO, :1405), same operation order

@@ -721,7 +721,19 @@ int validate(const Params& p) {
     // the reference's feature guards (src/fluxes.cpp:46-51, src/static_ic.cpp:26-29)
     if (p.lshr) return fail("unsupported feature enabled: LSHR");
     if (p.lrem) return fail("unsupported feature enabled: LREM");
-    if (p.rlax > 0.0) return fail("unsupported feature enabled: flux relaxation (RLAX)");
+    // flux relaxation (golden/fortran/3dmhdsub.f:1625-1771) is built for the
+    // ITC=0 / IZC=1 branch; in the ITC=1 / IZC=0 branch the last z rank's
+    // never-assigned FRADM(NZ) reaches every rank's correction (:1697-1760),
+    // and other boundary choices stop the Fortran ("Relax: check boundary
+    // temperature")
+    if (p.rlax > 0.0) {
+        if (!(p.itcon == 0 && p.izcon == 1))
+            return fail("flux relaxation (RLAX) is defined for ITCON=0 with IZCON=1 only");
+        if (p.ngrid != 0)
+            return fail("flux relaxation needs NGRID=0 (horizontal_mean: only NGRID=0 is supported)");
+        if (p.toff <= 0.0) return fail("flux relaxation needs TOFF > 0");
+        if (p.npez < 2) return fail("flux relaxation needs npez >= 2 (its z-rank exchanges)");
+    }
     // density diffusion (golden/fortran/3dmhdsub.f:1306-1380) is built for ID
     // 1 and 2; its horizontal mean exists for NGRID=0 only (src/communicate.cpp:190)
     if (p.id != 0 && p.id != 1 && p.id != 2)
@@ -835,6 +847,9 @@ int r3d_host_cfg(const r3d_host* h, r3d_hip_cfg* o) {
     o->dv = p.dv;
     o->zmax = p.zmax;
     o->id = p.id;
+    o->rlax = p.rlax;
+    o->tstart = p.tstart;
+    o->toff = p.toff;
     return 0;
 }
 
