@@ -100,9 +100,12 @@ PHYSICS = {
     "plain": [],
     "heat": ["--itcon", "2", "--tp", "1.5", "--zp", "20.0", "--hh", "4.0"],
     "density": ["--ngrid", "0", "--id", "1", "--dh", "0.4", "--dv", "0.3"],
+    "relax": ["--ngrid", "0", "--itcon", "0", "--izcon", "1", "--rlax", "0.5", "--toff", "0.5",
+              "--tstart", "-1"],
 }
 PHYSICS_NAMES = {"heat": "ITCON=2 embedded heat loss (TP 1.5, ZP 20, HH 4)",
-                 "density": "density diffusion ID=1 (DH 0.4, DV 0.3, uniform grid)"}
+                 "density": "density diffusion ID=1 (DH 0.4, DV 0.3, uniform grid)",
+                 "relax": "flux relaxation RLAX 0.5, TOFF 0.5 (ITCON=0, IZCON=1, uniform grid)"}
 
 
 def run_adjoint(a):
@@ -286,7 +289,7 @@ def main():
                     help="0: bit-exact stage kernel (default); 1: FMA-contracted stage kernel")
     ap.add_argument("--physics", default="plain", choices=sorted(PHYSICS),
                     help="plain (default); heat: the ITCON=2 embedded heat loss; density: "
-                         "density diffusion ID=1 (uniform grid)")
+                         "density diffusion ID=1 (uniform grid); relax: flux relaxation")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="before the run, time one device-to-device copy of this many GiB "

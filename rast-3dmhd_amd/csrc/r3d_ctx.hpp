@@ -57,6 +57,7 @@ struct r3d_hip_ctx {
     // ID=2 wall profile, TTM, FCONM and RLAX*TFAC*CORRECT, (npz+2) each, and
     // the relaxation state TFAC, TTM_MAX
     double* den_buf = nullptr;
+    double* ww2_buf = nullptr;  // flux relaxation: the convective flux WW2, one field
     double* stage_buf = nullptr;
     double* gstage = nullptr;  // one global field, (npx+2)(npy+2)(npz+2), allocated on first use
     // asynchronous snapshot (r3d_hip_snapshot_begin): X and Z as dense global

@@ -158,27 +158,19 @@ __global__ __launch_bounds__(DEN_PARTIAL_THREADS) void k_den_partial(const Geo g
 }
 
 // The convective flux WW2 = -RW*(2.5*(TT-TTM(K)) + 0.5*(UU*UU+VV*VV+WW*WW)),
-// UU = RU*(1/RO) (golden/fortran/3dmhdsub.f:1656-1666), summed like
-// k_den_partial: one sequential chain per (plane, y rank), rows j outer.
-__global__ __launch_bounds__(DEN_PARTIAL_THREADS) void k_ww2_partial(const Geo g, const State x,
-                                                                     const double* ttm,
-                                                                     double* part, int npey,
-                                                                     double cnt) {
-    const int t = blockIdx.x * DEN_PARTIAL_THREADS + threadIdx.x;
-    if (t >= (g.npz + 2) * npey) return;
-    const int K = t / npey, q = t % npey, nry = g.npy / npey;
-    const double tm = ttm[K];
-    double sum = 0.0;
-    for (int j = q * nry + 1; j <= (q + 1) * nry; ++j) {
-        const long n0 = g.at(1, j, K);
-        for (int i = 0; i < g.npx; ++i) {
-            const long n = n0 + i;
-            const double rw = x.q[QRW][n], ri = 1.0 / x.q[QRO][n];
-            const double uu = x.q[QRU][n] * ri, vv = x.q[QRV][n] * ri, ww = rw * ri;
-            sum += -rw * (2.5 * (x.q[QTT][n] - tm) + 0.5 * (uu * uu + vv * vv + ww * ww));
-        }
-    }
-    part[t] = sum / cnt;
+// UU = RU*(1/RO) (golden/fortran/3dmhdsub.f:1656-1666), of every interior
+// cell of planes 0..npz+1, into a field laid out like the state, so that
+// k_den_partial sums it with the reference's order.  One thread per cell:
+// the per-cell work (a division among it) runs at full occupancy instead of
+// inside the sequential sums.  grid = (row blocks, npy, npz + 2).
+__global__ __launch_bounds__(DEN_THREADS) void k_ww2_field(const Geo g, const State x,
+                                                           const double* ttm, double* w) {
+    const int i = 1 + blockIdx.x * DEN_THREADS + threadIdx.x, j = 1 + blockIdx.y, K = blockIdx.z;
+    if (i > g.npx) return;
+    const long n = g.at(i, j, K);
+    const double rw = x.q[QRW][n], ri = 1.0 / x.q[QRO][n];
+    const double uu = x.q[QRU][n] * ri, vv = x.q[QRV][n] * ri, ww = rw * ri;
+    w[n] = -rw * (2.5 * (x.q[QTT][n] - ttm[K]) + 0.5 * (uu * uu + vv * vv + ww * ww));
 }
 
 // The y ranks' partial means of one plane added in rank order.
@@ -196,11 +188,34 @@ __global__ __launch_bounds__(DEN_THREADS) void k_hmean_final(const double* part,
 // rank (nrz planes, the rank's lower ghost plane being the next rank's first)
 // FRADM, ALPHA and the trapezoid integral CORRECT, offset by the sum of the
 // ranks above's end values, and ftc = RLAX*TFAC*CORRECT for the interior.
-__global__ void k_rlax_chain(const DevScal* sc, int stage, double* st, const double* ttm,
-                             const double* fconm, double* ftc, const Metrics M, int npz, int npez,
-                             double hz, double repr, double theta, double rlax, double tstart,
-                             double toff) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// One block.  ALPHA depends on the plane only (a rank's lower ghost plane
+// takes the next rank's FRADM, which is that plane's own centred value), so
+// the block computes it for planes 1..npz+1 first; thread 0 then runs the
+// chain, reading ALPHA and ZEE eight planes ahead of the in-order sums.
+constexpr int RLX_THREADS = 256;
+__global__ __launch_bounds__(RLX_THREADS) void k_rlax_chain(
+    const DevScal* sc, int stage, double* st, const double* ttm, const double* fconm, double* alpha,
+    double* ftc, const Metrics M, int npz, int npez, double hz, double repr, double theta,
+    double rlax, double tstart, double toff) {
+    const double ftot = theta / repr;
+    for (int G = 1 + threadIdx.x; G <= npz + 1; G += RLX_THREADS) {
+        double fr;
+        if (G == npz + 1) {
+            fr = 0.0;  // the last rank's, never assigned in the Fortran; only in its unused end value
+        } else if (G == 1) {
+            fr = (-3.0 * ttm[1] + 4.0 * ttm[2] - ttm[3]) * hz * M.dzz[1] * M.rkapa[1] / repr;
+        } else if (G == npz) {
+            fr = (3.0 * ttm[npz] - 4.0 * ttm[npz - 1] + ttm[npz - 2]) * hz * M.dzz[npz] *
+                 M.rkapa[npz] / repr;
+        } else {
+            fr = (ttm[G + 1] - ttm[G - 1]) * hz * M.dzz[G] * M.rkapa[G] / repr;
+        }
+        double a = 1.0 - (fconm[G] + fr) / ftot;
+        alpha[G] = a / (1.0 + 4.0 * fabs(a));
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     const double dt = stage == 0 ? sc->dt_next : sc->dt;  // the step's dt
     const double timt = sc->timc + sc->timi;
     double tfac = st[0];
@@ -222,27 +237,32 @@ __global__ void k_rlax_chain(const DevScal* sc, int stage, double* st, const dou
         return;
     }
     const int nrz = npz / npez;
-    const double ftot = theta / repr;
-    double sum = 0.0;
+    double sum = 0.0;  // end values of the ranks above, added in rank order
     for (int pz = 0; pz < npez; ++pz) {
-        double corr = 0.0, aprev = 0.0;
-        for (int k = 1; k <= nrz + 1; ++k) {
-            const int G = pz * nrz + k;
-            double fr;
-            if (k == nrz + 1 && pz == npez - 1) {
-                fr = 0.0;  // never assigned in the Fortran; only in this rank's unused end value
-            } else if (pz == 0 && k == 1) {
-                fr = (-3.0 * ttm[1] + 4.0 * ttm[2] - ttm[3]) * hz * M.dzz[1] * M.rkapa[1] / repr;
-            } else if (pz == npez - 1 && k == nrz) {
-                fr = (3.0 * ttm[npz] - 4.0 * ttm[npz - 1] + ttm[npz - 2]) * hz * M.dzz[npz] *
-                     M.rkapa[npz] / repr;
-            } else {
-                fr = (ttm[G + 1] - ttm[G - 1]) * hz * M.dzz[G] * M.rkapa[G] / repr;
+        const int G0 = pz * nrz;
+        double corr = 0.0;
+        // CORRECT(k) = CORRECT(k-1) + 0.5*(ALPHA(k)+ALPHA(k-1))*(ZEE(k)-ZEE(k-1)),
+        // k = 2..nrz+1 of the rank (global G0+k)
+        int k = 2;
+        constexpr int U = 8;
+        for (; k + U - 1 <= nrz + 1; k += U) {
+            double ap[U + 1], zp[U + 1];
+#pragma unroll
+            for (int u = 0; u <= U; ++u) {
+                ap[u] = alpha[G0 + k - 1 + u];
+                zp[u] = M.zee[G0 + k - 1 + u];
             }
-            double a = 1.0 - (fconm[G] + fr) / ftot;
-            a = a / (1.0 + 4.0 * fabs(a));
-            corr = k == 1 ? 0.0 : corr + 0.5 * (a + aprev) * (M.zee[G] - M.zee[G - 1]);
-            aprev = a;
+            if (k == 2) ftc[G0 + 1] = rt * (corr + sum);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                corr = corr + 0.5 * (ap[u + 1] + ap[u]) * (zp[u + 1] - zp[u]);
+                if (k + u <= nrz) ftc[G0 + k + u] = rt * (corr + sum);
+            }
+        }
+        if (k == 2) ftc[G0 + 1] = rt * (corr + sum);
+        for (; k <= nrz + 1; ++k) {
+            const int G = G0 + k;
+            corr = corr + 0.5 * (alpha[G] + alpha[G - 1]) * (M.zee[G] - M.zee[G - 1]);
             if (k <= nrz) ftc[G] = rt * (corr + sum);
         }
         sum = sum + corr;  // this rank's end value, CORRECT(NZ)
@@ -458,7 +478,8 @@ int launch_relax_prep(r3d_hip_ctx* c, int stage) {
     double* ttm = c->den_buf + nkz * (f.npey + 4);
     double* fconm = ttm + nkz;
     double* ftc = fconm + nkz;
-    double* st = c->den_buf + nkz * (f.npey + 7);
+    double* alpha = ftc + nkz;
+    double* st = c->den_buf + nkz * (f.npey + 8);
     State s;
     for (int q = 0; q < NXF; ++q) s.q[q] = c->xs[c->cur][q];
     const double cnt = (double)(f.npx * f.npy);
@@ -468,13 +489,16 @@ int launch_relax_prep(r3d_hip_ctx* c, int stage) {
                        (const double*)s.q[QTT], part, f.npey, cnt);
     hipLaunchKernelGGL(k_hmean_final, gf, dim3(DEN_THREADS), 0, c->stream, (const double*)part, ttm,
                        nk, f.npey);
-    hipLaunchKernelGGL(k_ww2_partial, gp, dim3(DEN_PARTIAL_THREADS), 0, c->stream, c->g, s,
-                       (const double*)ttm, part, f.npey, cnt);
+    hipLaunchKernelGGL(k_ww2_field, dim3((f.npx + DEN_THREADS - 1) / DEN_THREADS, f.npy, nk),
+                       dim3(DEN_THREADS), 0, c->stream, c->g, s, (const double*)ttm, c->ww2_buf);
+    hipLaunchKernelGGL(k_den_partial, gp, dim3(DEN_PARTIAL_THREADS), 0, c->stream, c->g,
+                       (const double*)c->ww2_buf, part, f.npey, cnt);
     hipLaunchKernelGGL(k_hmean_final, gf, dim3(DEN_THREADS), 0, c->stream, (const double*)part,
                        fconm, nk, f.npey);
-    hipLaunchKernelGGL(k_rlax_chain, dim3(1), dim3(64), 0, c->stream, (const DevScal*)c->scal, stage,
-                       st, (const double*)ttm, (const double*)fconm, ftc, c->M, f.npz, f.npez, f.hz,
-                       f.repr, f.theta, f.rlax, f.tstart, f.toff);
+    hipLaunchKernelGGL(k_rlax_chain, dim3(1), dim3(RLX_THREADS), 0, c->stream,
+                       (const DevScal*)c->scal, stage, st, (const double*)ttm, (const double*)fconm,
+                       alpha, ftc, c->M, f.npz, f.npez, f.hz, f.repr, f.theta, f.rlax, f.tstart,
+                       f.toff);
     HIPCHK(hipGetLastError());
     return R3D_OK;
 }
@@ -707,7 +731,7 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
                         2 * ((size_t)(f.npx + 2) + (size_t)(f.npy + 2)) + 2 * (size_t)(f.npz + 2);
     if ((rc = dev_alloc(c, &c->metric_buf, nmet))) return fail(rc);
     if (f.id != 0 || f.rlax > 0.0) {
-        const size_t nk = (size_t)f.npz + 2, n = nk * ((size_t)f.npey + 7) + 2;
+        const size_t nk = (size_t)f.npz + 2, n = nk * ((size_t)f.npey + 8) + 2;
         if ((rc = dev_alloc(c, &c->den_buf, n))) return fail(rc);
         // zero rows the run does not compute; TFAC = 1, TTM_MAX = -1e99
         // (DATA statements, golden/fortran/3dmhdsub.f:1258-1259)
@@ -722,6 +746,7 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
         c->M.dtv = c->M.dth + nk;
         c->M.ftc = c->M.rom + 6 * nk;
     }
+    if (f.rlax > 0.0 && (rc = dev_alloc(c, &c->ww2_buf, field_elems(c->g)))) return fail(rc);
     const size_t stage_elems = (size_t)std::max<long>((long)c->nx * c->ny * c->nz,
                                                       (long)(f.npx + 2) * (f.npy + 2));
     if ((rc = dev_alloc(c, &c->stage_buf, stage_elems))) return fail(rc);
@@ -745,6 +770,7 @@ int r3d_hip_destroy(r3d_hip_ctx* c) {
     if (c->scal) (void)hipFree(c->scal);
     if (c->metric_buf) (void)hipFree(c->metric_buf);
     if (c->den_buf) (void)hipFree(c->den_buf);
+    if (c->ww2_buf) (void)hipFree(c->ww2_buf);
     if (c->stage_buf) (void)hipFree(c->stage_buf);
     if (c->gstage) (void)hipFree(c->gstage);
     if (c->copy_stream) {
