@@ -71,6 +71,12 @@ typedef struct r3d_hip_cfg {
     double rlax, tstart, toff;  /* flux relaxation (golden/fortran/3dmhdsub.f:1625-1771):
                                    rlax > 0 turns it on; ITCON=0 with IZCON=1, NGRID=0,
                                    npez >= 2 and toff > 0 only */
+    /* Stage-kernel tiling (results do not depend on it; 0 = chosen by the
+     * library).  tile_rows: y rows per workgroup tile, 8 or 9 for hydro (8
+     * with itcon 2, id or rlax), 5 with lmag.  zchunk: z planes per
+     * workgroup, 8, 16, 32, 64 or 128.  No reference counterpart: the
+     * reference GPU launches one thread per cell (src/gpu.cpp). */
+    int tile_rows, zchunk;
 } r3d_hip_cfg;
 
 /* Global 1-D metrics.  x arrays have npx+2 entries, y arrays npy+2, z arrays
@@ -154,6 +160,11 @@ int r3d_hip_kernel_stats(r3d_hip_ctx* ctx, int cls, long* launches, double* tota
 
 /* Device bytes held by the context. */
 int r3d_hip_memory(r3d_hip_ctx* ctx, unsigned long long* bytes);
+
+/* Stage-kernel tiling in use: out = {tile rows, z chunk, x tiles, y tiles,
+ * z chunks}.  No reference counterpart (the reference GPU launches one
+ * thread per cell). */
+int r3d_hip_tiling(r3d_hip_ctx* ctx, int out[5]);
 
 /* Thread-local message of the last failed call. */
 const char* r3d_hip_last_error(void);

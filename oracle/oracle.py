@@ -39,6 +39,8 @@ def lib():
             getattr(L, fn).restype = ctypes.c_int
         L.o_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.o_step.restype = ctypes.c_int
+        L.o_step_dt.argtypes = [ctypes.c_void_p]
+        L.o_step_dt.restype = ctypes.c_int
         L.o_fluxes.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.o_fluxes.restype = ctypes.c_int
         L.o_bcon.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -119,6 +121,10 @@ class Sim:
     def step(self, nsteps=1, nstages=3):
         for _ in range(nsteps):
             self._check(lib().o_step(self.h, nstages))
+
+    def step_dt(self):
+        """STEP's time-step limits alone: uu, vv (LMAG), ww1..ww3, umach, dt."""
+        self._check(lib().o_step_dt(self.h))
 
     def fluxes(self, rank):
         self._check(lib().o_fluxes(self.h, rank))

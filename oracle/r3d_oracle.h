@@ -71,6 +71,11 @@ int o_init(o_sim* s);
 
 /* One RK3 step, or only the first nstages (1..3) stages (src/step.cpp:114). */
 int o_step(o_sim* s, int nstages);
+/* The time-step limits of STEP alone (reference src/step.cpp:121-229): UU,
+ * VV (LMAG), WW1..WW3 over every rank's interior, umach and dt, without the
+ * RK stages.  o_step begins with it.  Pinned against the reference's own
+ * STEP with LMAG (tests/test_lmag_pins.py). */
+int o_step_dt(o_sim* s);
 
 /* Standalone pieces for invariant tests. */
 int o_fluxes(o_sim* s, int rank);      /* src/fluxes.cpp:44 */

@@ -19,7 +19,7 @@ CXXFLAGS := -O2 -std=c++17 -ffp-contract=off -fopenmp -I$(REF)/include
 SRCS := params topology grid ode ran2 static_ic fluxes bcon communicate step
 OBJS := $(addprefix $(OUT)/obj/,$(addsuffix .o,$(SRCS)))
 
-all: $(OUT)/3dmhd $(OUT)/3dmhd_nodump $(OUT)/hmean
+all: $(OUT)/3dmhd $(OUT)/3dmhd_nodump $(OUT)/hmean $(OUT)/stepdt
 
 $(OUT)/obj/%.o: $(REF)/src/%.cpp
 	@mkdir -p $(OUT)/obj
@@ -48,3 +48,13 @@ $(OUT)/obj/ref_hmean.o: $(HERE)/ref_hmean.cpp
 
 $(OUT)/hmean: $(OUT)/obj/ref_hmean.o $(OBJS)
 	$(MPICXX) $(CXXFLAGS) -o $@ $^ -lm
+
+# the reference's STEP time-step limits (LMAG included), driven by
+# oracle/ref_stepdt.cpp; the link sends STEP's first FLUXES call there
+FLUXES_SYM := _ZN3r3d6fluxesERKNS_6ParamsERKNS_7DerivedERKNS_8TopologyEPKdSA_RNS_8SimStateE
+$(OUT)/obj/ref_stepdt.o: $(HERE)/ref_stepdt.cpp
+	@mkdir -p $(OUT)/obj
+	$(MPICXX) $(CXXFLAGS) -c -o $@ $<
+
+$(OUT)/stepdt: $(OUT)/obj/ref_stepdt.o $(OBJS)
+	$(MPICXX) $(CXXFLAGS) -Wl,--wrap=$(FLUXES_SYM) -o $@ $^ -lm

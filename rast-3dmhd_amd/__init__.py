@@ -44,7 +44,8 @@ class Cfg(ctypes.Structure):
                  "rm", "orm", "obeta", "omx", "omz",
                  "tp", "tc", "hh", "xp", "yp", "dzf", "tb", "zp", "qfh", "dh", "dv", "zmax")] + \
                [("id", ctypes.c_int)] + \
-               [(n, ctypes.c_double) for n in ("rlax", "tstart", "toff")]
+               [(n, ctypes.c_double) for n in ("rlax", "tstart", "toff")] + \
+               [(n, ctypes.c_int) for n in ("tile_rows", "zchunk")]
 
 
 METRIC_NAMES = ["exx", "dxxdx", "d2xxdx2", "ddx", "wyy", "dyydy", "d2yydy2", "ddy",
@@ -91,6 +92,7 @@ def load():
         "r3d_hip_kernel_stats": [vp, ip, ctypes.POINTER(ctypes.c_long),
                                  ctypes.POINTER(ctypes.c_double)],
         "r3d_hip_memory": [vp, ctypes.POINTER(ctypes.c_ulonglong)],
+        "r3d_hip_tiling": [vp, ctypes.POINTER(ip)],
         # batched ensemble (include/r3d_ens.h)
         "r3d_ens_create": [ip, ctypes.POINTER(Cfg), ctypes.POINTER(vp)],
         "r3d_ens_destroy": [vp],
@@ -151,9 +153,18 @@ class HipSim:
         self.ny = cfg.npy // cfg.npey + 2
         self.nz = cfg.npz // cfg.npez + 2
         self.nf = 8 if cfg.lmag else 5
+        self._writer = None  # dump thread reading the snapshot buffers (Run.dump_async)
+
+    def _join_writer(self):
+        """Wait for a dump thread still reading this context's snapshot: the
+        next snapshot_begin overwrites its pinned buffer and destroy frees it."""
+        w, self._writer = getattr(self, "_writer", None), None
+        if w is not None and w.is_alive():
+            w.join()
 
     def close(self):
         if self.h:
+            self._join_writer()
             load().r3d_hip_destroy(self.h)
             self.h = None
 
@@ -225,7 +236,9 @@ class HipSim:
         _check(load().r3d_hip_step(self.h, nsteps, nstages))
 
     def snapshot_begin(self):
-        """Start an asynchronous snapshot of the current state (include/r3d_hip.h)."""
+        """Start an asynchronous snapshot of the current state (include/r3d_hip.h).
+        A dump thread still writing the previous snapshot is joined first."""
+        self._join_writer()
         _check(load().r3d_hip_snapshot_begin(self.h))
 
     def snapshot_rank(self, rank, names=None):
@@ -256,6 +269,12 @@ class HipSim:
         b = ctypes.c_ulonglong()
         _check(load().r3d_hip_memory(self.h, ctypes.byref(b)))
         return b.value
+
+    def tiling(self):
+        """Stage tiling in use: (tile rows, z chunk, x tiles, y tiles, z chunks)."""
+        out = (ctypes.c_int * 5)()
+        _check(load().r3d_hip_tiling(self.h, out))
+        return tuple(out)
 
 
 class Ensemble:
@@ -615,7 +634,9 @@ class Run:
         and write every virtual rank's gstate dump
         (directory/gstate.<tag>.<rank>.<nit>, the reference's names) on a
         background thread while the caller keeps stepping.  Returns the
-        started thread; join it before the next dump_async on this sim.  A run
+        started thread.  The next snapshot on this sim (another dump_async or
+        snapshot_begin) and sim.close() join it first, so a dump is never
+        torn by a later snapshot or by freeing the buffers under it.  A run
         started with --nstart <nit> --dumpdir <directory> continues from it."""
         import threading
         sim.snapshot_begin()
@@ -632,14 +653,18 @@ class Run:
 
         t = threading.Thread(target=write, daemon=True)
         t.errors = errors
+        sim._writer = t
         t.start()
         return t
 
-    def make_sim(self, variant=0):
+    def make_sim(self, variant=0, tile_rows=0, zchunk=0):
         """Create the device context and upload this run's metrics, state and
-        starting scalars."""
+        starting scalars.  tile_rows / zchunk override the stage-kernel tiling
+        (0 = the library's choice; include/r3d_hip.h)."""
         cfg = self.cfg()
         cfg.variant = variant
+        cfg.tile_rows = tile_rows
+        cfg.zchunk = zchunk
         sim = HipSim(cfg)
         sim.set_metrics(self.metrics())
         sim.upload_global(self.state(copy=False))

@@ -106,7 +106,9 @@ class _KineticEnergy(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gJ):
-        grads, dre, dth = ctx.adj.backward()
+        adj, ctx.adj = ctx.adj, None
+        grads, dre, dth = adj.backward()
+        adj.close()  # frees its checkpoints and segment buffers on the device now
         g = torch.from_numpy(grads["tt"][1:-1, 1:-1, 1:-1].copy()).to(ctx.dev)
         return (gJ * g, gJ * torch.tensor(dre, dtype=torch.float64, device=ctx.dev),
                 gJ * torch.tensor(dth, dtype=torch.float64, device=ctx.dev), None)
@@ -155,7 +157,7 @@ class _KineticEnergyState(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gJ):
-        adj = ctx.adj
+        adj, ctx.adj = ctx.adj, None
         grads, dre, dth = adj.backward()
         dtb = adj.dJ_dtb()
         adj.close()

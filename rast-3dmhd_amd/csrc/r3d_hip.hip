@@ -665,6 +665,17 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     }
     if (f.variant != 0 && f.variant != 1)
         return set_err(R3D_EUNSUP, "variant %d: 0 (bit-exact) and 1 (FMA) are built", f.variant);
+    const bool feature = f.itcon == 2 || f.id != 0 || f.rlax > 0.0;
+    if (f.tile_rows != 0) {
+        const bool ok = f.lmag ? f.tile_rows == TILE_Y_MAG
+                               : feature ? f.tile_rows == 8 : (f.tile_rows == 8 || f.tile_rows == TILE_Y);
+        if (!ok)
+            return set_err(R3D_EINVAL, "tile_rows %d: built are %s", f.tile_rows,
+                           f.lmag ? "5 (LMAG)" : feature ? "8 (itcon 2, id, rlax)" : "8 and 9");
+    }
+    if (f.zchunk != 0 && f.zchunk != 8 && f.zchunk != 16 && f.zchunk != 32 && f.zchunk != 64 &&
+        f.zchunk != 128)
+        return set_err(R3D_EINVAL, "zchunk %d: one of 8, 16, 32, 64, 128", f.zchunk);
     HIPCHK(hipSetDevice(f.device));
 
     auto* c = new r3d_hip_ctx();
@@ -684,8 +695,8 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     fill_consts(c);
 
     // stage tiling: 64 x ty columns, z chunks sized for >= ~8 blocks per CU
-    c->ty = f.lmag ? TILE_Y_MAG
-                   : (f.itcon == 2 || f.id != 0 || f.rlax > 0.0) ? 8 : hydro_tile_rows(f.npy);
+    c->ty = f.lmag ? TILE_Y_MAG : feature ? 8 : hydro_tile_rows(f.npy);
+    if (f.tile_rows) c->ty = f.tile_rows;
     c->ntx = (f.npx + 63) / 64;
     c->nty = (f.npy + c->ty - 1) / c->ty;
     c->kc = 8;
@@ -695,6 +706,7 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
             break;
         }
     }
+    if (f.zchunk) c->kc = f.zchunk;
     c->nkc = (f.npz + c->kc - 1) / c->kc;
     c->nblocks = c->ntx * c->nty * c->nkc;
     c->grid = (c->nblocks + 7) / 8 * 8;
@@ -1136,6 +1148,16 @@ int r3d_hip_kernel_stats(r3d_hip_ctx* c, int cls, long* launches, double* total_
 int r3d_hip_memory(r3d_hip_ctx* c, unsigned long long* bytes) {
     if (!c || !bytes) return set_err(R3D_EINVAL, "null argument");
     *bytes = c->bytes;
+    return R3D_OK;
+}
+
+int r3d_hip_tiling(r3d_hip_ctx* c, int out[5]) {
+    if (!c || !out) return set_err(R3D_EINVAL, "null argument");
+    out[0] = c->ty;
+    out[1] = c->kc;
+    out[2] = c->ntx;
+    out[3] = c->nty;
+    out[4] = c->nkc;
     return R3D_OK;
 }
 

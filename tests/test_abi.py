@@ -65,6 +65,18 @@ def test_create_rejects_bad_config_without_a_device():
     assert L.r3d_ens_destroy(None) == 0
 
 
+@pytest.mark.parametrize("bad", [(7, 0, 0), (5, 0, 0), (9, 0, 1), (0, 24, 0), (0, 256, 0)])
+def test_create_rejects_unbuilt_tilings_without_a_device(bad):
+    """tile_rows / zchunk outside the built kernels are configuration errors."""
+    L = pkg.load()
+    cfg = pkg.Cfg()
+    cfg.npx, cfg.npy, cfg.npz, cfg.npey, cfg.npez = 16, 16, 32, 2, 2
+    cfg.tile_rows, cfg.zchunk, cfg.lmag = bad
+    h = ctypes.c_void_p()
+    assert L.r3d_hip_create(ctypes.byref(cfg), ctypes.byref(h)) == -1 and not h.value
+    assert b"tile_rows" in L.r3d_hip_last_error() or b"zchunk" in L.r3d_hip_last_error()
+
+
 def test_ensemble_rejects_mixed_members_without_a_device():
     L = pkg.load()
     cfgs = (pkg.Cfg * 2)()

@@ -1,0 +1,196 @@
+"""The HIP time step at the workload sizes, with the tiling the library picks there.
+
+* Hydro 504x504x640 on 4x2 virtual ranks: the library picks 128-plane z
+  chunks, eight 64-lane x tiles (the last one 56 wide) and 63 eight-row y
+  tiles, the tiling of the 504x504x2048 production grid.  After 3 steps every
+  rank's primaries (ghosts included), RK accumulators and scalars must hash
+  as the reference CPU build's dumps (tests/golden/ref_prod.json).
+* Hydro 504x504x2048 on 4x2 (the production grid itself, too large for a
+  reference run in 64 GB of host memory): properties that hold bit for bit
+  whatever the size, the tiling invariance of the result, the x-roll
+  invariance of a run without the plume, and the error of the FMA variant
+  after 100 steps.
+* Rotating MHD 252x252x512 on 2x4 (the MHD workload): after 1, 20 and 200
+  steps every rank must hash as the oracle (tests/golden/oracle_cfg1.json,
+  tests/golden/make_oracle_cfg1.py); the reference cannot step with LMAG.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import load_package
+
+pkg = load_package()
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+X5 = ["ru", "rv", "rw", "ro", "tt"]
+Z5 = ["zru", "zrv", "zrw", "zro", "ztt"]
+PROD = ["--npx", "504", "--npy", "504", "--npz", "2048", "--npey", "4", "--npez", "2"]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:20]
+
+
+def load(name):
+    with open(os.path.join(HERE, "golden", name)) as f:
+        return json.load(f)
+
+
+def test_production_tiling_matches_reference_504x504x640():
+    cfg = load("ref_prod.json")["p504x640_4x2"]
+    args = cfg["args"] + ["--npey", str(cfg["npey"]), "--npez", str(cfg["npez"])]
+    say(f"[gpu] production tiling: {' '.join(args)}")
+    run = pkg.Run(args)
+    sim = run.make_sim()
+    run.close()
+    ty, kc, ntx, nty, nkc = sim.tiling()
+    assert (ty, kc, ntx, nty, nkc) == (8, 128, 8, 63, 5)
+    done = 0
+    for st in sorted(int(s) for s in cfg["dumps"]):
+        sim.step(st - done)
+        done = st
+        bad = []
+        for r, want in enumerate(cfg["dumps"][str(st)]):
+            got = sim.download_rank(r, X5 + Z5)
+            for n in X5 + Z5:
+                if digest(got[n]) != want["fields"][n]:
+                    bad.append((r, n))
+            del got
+        sc = sim.scalars()
+        for k in ("dt", "timc", "timt", "umach"):
+            if sc[k].hex() != cfg["dumps"][str(st)][0]["scalars"][k]:
+                bad.append((k, sc[k]))
+        say(f"[gpu]   step {st}: {len(bad)} mismatches")
+        assert not bad, f"step {st}: {bad[:8]}"
+    sim.close()
+
+
+def run_global(run, steps, **kw):
+    sim = run.make_sim(**kw)
+    sim.step(steps)
+    out = sim.download_global(X5)
+    sc = sim.scalars()
+    til = sim.tiling()
+    sim.close()
+    return out, sc, til
+
+
+def same_bits(a, b):
+    return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_production_grid_tiling_invariance():
+    """504x504x2048: the library's tiling (8 rows, 128-plane chunks) and a
+    forced one (9 rows, 32-plane chunks) give the same state bit for bit."""
+    say("[gpu] 504x504x2048 tiling invariance, 3 steps")
+    run = pkg.Run(PROD)
+    a, sa, ta = run_global(run, 3)
+    assert ta[:2] == (8, 128), ta
+    b, sb, tb = run_global(run, 3, tile_rows=9, zchunk=32)
+    assert tb[:2] == (9, 32), tb
+    run.close()
+    for n in X5:
+        assert same_bits(a[n], b[n]), n
+    assert sa == sb
+
+
+def test_production_grid_x_roll_invariance():
+    """504x504x2048 on a uniform grid (ngrid 0) with a plume-free top (tp = 0):
+    x is periodic and uniform,
+    so rolling the initial state by 37 cells in x rolls the state after 3
+    steps by 37 cells, bit for bit: the 56-lane last x tile and the tile
+    seams shift across the data."""
+    say("[gpu] 504x504x2048 x-roll invariance, 3 steps")
+    run = pkg.Run(PROD + ["--tp", "0", "--ngrid", "0"])
+    st = run.state()
+    rolled = {}
+    for n in X5:
+        a = st[n].copy()
+        a[:, :, 1:-1] = np.roll(st[n][:, :, 1:-1], 37, axis=2)
+        a[:, :, 0], a[:, :, -1] = a[:, :, -2], a[:, :, 1]
+        rolled[n] = a
+    sim = run.make_sim()
+    sim.step(3)
+    ref = sim.download_global(X5)
+    sim.close()
+    sim = pkg.HipSim(run.cfg())
+    sim.set_metrics(run.metrics())
+    sim.upload_global(rolled)
+    sim.set_scalars(**run.scalars())
+    del rolled
+    sim.step(3)
+    got = sim.download_global(X5)
+    sim.close()
+    run.close()
+    for n in X5:
+        want = np.roll(ref[n][:, :, 1:-1], 37, axis=2)
+        assert same_bits(got[n][:, :, 1:-1], want), n
+
+
+def test_production_grid_fma_error_after_100_steps():
+    """504x504x2048, 100 steps: the FMA variant against the bit-exact path,
+    normwise max|a-b|/max|b| <= 1e-12 per primary; the elementwise relative
+    error is stated (positive fields: <= 1e-12 everywhere)."""
+    say("[gpu] 504x504x2048 FMA variant, 100 steps")
+    run = pkg.Run(PROD)
+    a, sa, _ = run_global(run, 100)
+    b, sb, _ = run_global(run, 100, variant=1)
+    run.close()
+    for n in X5:
+        x, y = a[n][1:-1, 1:-1, 1:-1], b[n][1:-1, 1:-1, 1:-1]
+        scale = np.abs(x).max()
+        d = np.abs(y - x)
+        norm = float(d.max() / scale)
+        elem = float((d / np.maximum(np.abs(x), 1e-300)).max())
+        floor = float((d / np.maximum(np.abs(x), 1e-6 * scale)).max())
+        say(f"[gpu]   {n}: normwise {norm:.3e}, elementwise {elem:.3e}, "
+            f"elementwise with |b| floored at 1e-6 max|b| {floor:.3e}")
+        assert norm <= 1e-12, (n, norm)
+        if n in ("ro", "tt"):
+            assert elem <= 1e-12, (n, elem)
+    assert abs(sa["timc"] - sb["timc"]) <= 1e-12 * sa["timc"]
+
+
+def test_rotating_mhd_workload_matches_oracle_252x252x512():
+    cfg = load("oracle_cfg1.json")
+    args = cfg["args"] + ["--npey", str(cfg["npey"]), "--npez", str(cfg["npez"])]
+    say(f"[gpu] rotating MHD workload: {' '.join(args)}")
+    run = pkg.Run(args)
+    sim = run.make_sim()
+    run.close()
+    ty, kc = sim.tiling()[:2]
+    assert (ty, kc) == (5, 32), (ty, kc)
+    X = X5 + ["bx", "by", "bz"]
+    Z = ["z" + n for n in X]
+    done = 0
+    for st in sorted(int(s) for s in cfg["dumps"]):
+        sim.step(st - done)
+        done = st
+        rec = cfg["dumps"][str(st)]
+        bad = []
+        for r, want in enumerate(rec["ranks"]):
+            got = sim.download_rank(r, X + Z)
+            for n in X:
+                if digest(got[n]) != want[n]:
+                    bad.append((r, n))
+            for n in Z:
+                if digest(got[n][1:-1, 1:-1, 1:-1]) != want[n]:
+                    bad.append((r, n))
+        sc = sim.scalars()
+        for k in ("dt", "timc", "timt", "umach"):
+            if sc[k].hex() != rec["scalars"][k]:
+                bad.append((k, sc[k]))
+        assert sc["nit"] == rec["nit"]
+        say(f"[gpu]   step {st}: {len(bad)} mismatches")
+        assert not bad, f"step {st}: {bad[:8]}"
+    sim.close()
