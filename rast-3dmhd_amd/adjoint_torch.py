@@ -45,7 +45,17 @@ class KineticEnergyRun:
         self._key = None
         self._adj = None
 
+    def close(self):
+        """Free the cached adjoint context (device checkpoints and segment
+        buffers) and the host setup."""
+        if self._adj is not None:
+            self._adj.close()
+            self._adj, self._key = None, None
+        self.run.close()
+
     def adjoint(self, re, theta):
+        """The one adjoint context this problem keeps on the device: reused
+        while (Re, theta) stay the same, closed and replaced when they change."""
         key = (re, theta)
         if key != self._key:
             if self._adj is not None:
@@ -106,9 +116,10 @@ class _KineticEnergy(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gJ):
+        # the context belongs to prob (prob.adjoint caches one, reused while
+        # Re and theta stay the same; prob.close() frees it)
         adj, ctx.adj = ctx.adj, None
         grads, dre, dth = adj.backward()
-        adj.close()  # frees its checkpoints and segment buffers on the device now
         g = torch.from_numpy(grads["tt"][1:-1, 1:-1, 1:-1].copy()).to(ctx.dev)
         return (gJ * g, gJ * torch.tensor(dre, dtype=torch.float64, device=ctx.dev),
                 gJ * torch.tensor(dth, dtype=torch.float64, device=ctx.dev), None)
