@@ -32,11 +32,22 @@ BYTES_PER_CELL_STAGE = 400.0 / 3.0   # hydro: 10 passes x 5 fields x 8 B per ste
 BYTES_PER_CELL_STAGE_MHD = 640.0 / 3.0  # MHD: 10 passes x 8 fields x 8 B per step
 MHD_FLAGS = ["--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2",
              "--bzp", "10", "--lrot", "1", "--ry", "0.1", "--ang", "0"]
-CPU_CORES = 16                        # host cores available per GPU
+CPU_CORES = 16                        # the host's CPU share per GPU on the MI355X boxes
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def cpu_baseline(npx, npy, npey, npez, cores):
@@ -44,31 +55,39 @@ def cpu_baseline(npx, npy, npey, npez, cores):
     ref = os.path.join(ROOT, "oracle", "_ref", "3dmhd_nodump")
     mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
     nsample = 256  # reduced z extent: ~15 GB of reference state instead of ~115 GB
-    steps = 4
+    # steps 2..14 are timed from the reference's WALL log (0.1 s resolution):
+    # about 12 s at this size, so the resolution is under 1%
+    steps = 14
     if os.path.exists(ref) and os.path.exists(mpirun):
         np_ = npey * npez
-        env = dict(os.environ, OMP_NUM_THREADS=str(max(1, cores // np_)))
-        cmd = [mpirun, "-np", str(np_), ref, "--npx", str(npx), "--npy", str(npy),
-               "--npz", str(nsample), "--npey", str(npey), "--ntotal", str(steps),
-               "--nstep0", "1"]
-        log(f"[bench] cpu baseline: {' '.join(cmd)} (OMP_NUM_THREADS={env['OMP_NUM_THREADS']})")
-        try:
-            p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=300,
-                               cwd="/tmp")
+        per = max(1, cores // np_)
+        env = dict(os.environ, OMP_NUM_THREADS=str(per), OMP_PROC_BIND="true",
+                   OMP_PLACES="cores")
+        args = [ref, "--npx", str(npx), "--npy", str(npy), "--npz", str(nsample),
+                "--npey", str(npey), "--ntotal", str(steps), "--nstep0", "1"]
+        # each rank bound to its own `per` cores, its OpenMP threads one per core
+        for bind in (["-bind-to", f"core:{per}"], []):
+            cmd = [mpirun] + bind + ["-np", str(np_)] + args
+            log(f"[bench] cpu baseline: {' '.join(cmd)} (OMP_NUM_THREADS={per})")
+            try:
+                p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=400,
+                                   cwd="/tmp")
+            except (OSError, subprocess.TimeoutExpired) as e:
+                log(f"[bench] reference run unavailable: {e}")
+                break
             walls = {int(m.group(1)): float(m.group(2)) for m in
                      re.finditer(r"NIT (\d+) .*WALL ([0-9.]+)s", p.stderr)}
             if p.returncode == 0 and 1 in walls and steps in walls:
                 per_step = (walls[steps] - walls[1]) / (steps - 1)
                 cells = npx * npy * nsample
                 return {"value": cells * 3 / per_step, "unit": "cell-updates/s",
-                        "cores": (np_ * int(env["OMP_NUM_THREADS"])), "kind": "reference",
-                        "sample": f"reference CPU build (MPI {npey}x{npez} ranks x "
-                                  f"{env['OMP_NUM_THREADS']} OpenMP threads), "
-                                  f"{npx}x{npy}x{nsample} (z reduced from 2048), "
-                                  f"steps 2..{steps} timed from its own WALL log"}
+                        "cores": np_ * per, "kind": "reference",
+                        "sample": f"reference CPU build (MPI {npey}x{npez} ranks x {per} OpenMP "
+                                  f"threads{', each rank bound to its own cores' if bind else ''}"
+                                  f") on {cpu_model()}, {npx}x{npy}x{nsample} (z reduced from "
+                                  f"2048), steps 2..{steps} ({walls[steps] - walls[1]:.1f} s) "
+                                  f"timed from its own WALL log"}
             log(f"[bench] reference run failed (rc={p.returncode}): {p.stderr[-400:]}")
-        except (OSError, subprocess.TimeoutExpired) as e:
-            log(f"[bench] reference run unavailable: {e}")
     # fallback: the C oracle (same sweeps, OpenMP) on a smaller sample
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     os.environ["OMP_NUM_THREADS"] = str(cores)

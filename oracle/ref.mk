@@ -58,3 +58,35 @@ $(OUT)/obj/ref_stepdt.o: $(HERE)/ref_stepdt.cpp
 
 $(OUT)/stepdt: $(OUT)/obj/ref_stepdt.o $(OBJS)
 	$(MPICXX) $(CXXFLAGS) -Wl,--wrap=$(FLUXES_SYM) -o $@ $^ -lm
+
+# The reference's own driver (src/main.cpp) with its --backend gpu loop
+# (src/main.cpp:234-275), linked with the INTEGRATION.md shim and the product
+# library: the reference calls the library through its include/gpu.hpp
+# surface.  main.cpp is compiled with R3D_HAVE_GPU and cudaSetDevice defined
+# away (the library opens cfg.device itself); no CUDA is involved.  The shim
+# source is the code block of INTEGRATION.md, extracted into _ref/dropin/.
+ROOT   := $(abspath $(HERE)/..)
+DROPIN := $(OUT)/dropin
+all: $(OUT)/3dmhd_gpu
+
+$(DROPIN)/gpu_hip.cpp: $(ROOT)/INTEGRATION.md
+	@mkdir -p $(DROPIN)
+	python3 -c "import re,sys; b=re.findall(r'\`\`\`(?:cpp|c\+\+)\n(.*?)\`\`\`', open(sys.argv[1]).read(), re.S); assert len(b)==1; open(sys.argv[2],'w').write(b[0])" $< $@
+
+$(OUT)/obj/gpu_hip.o: $(DROPIN)/gpu_hip.cpp $(ROOT)/include/r3d_hip.h $(ROOT)/include/r3d_gpu.hpp
+	@mkdir -p $(OUT)/obj
+	$(MPICXX) $(CXXFLAGS) -I$(ROOT)/include -c -o $@ $<
+
+$(OUT)/obj/main_gpu.o: $(REF)/src/main.cpp
+	@mkdir -p $(OUT)/obj
+	$(MPICXX) $(CXXFLAGS) -DR3D_DUMP_MODE=2 -DR3D_HAVE_GPU '-DcudaSetDevice(d)=((void)(d))' -c -o $@ $<
+
+# Linked by g++ with MPICH's libraries named by path: the ROCm runtime needs
+# the system libstdc++, which the wrapper's -L/rpath of its own lib directory
+# (an older libstdc++) would hide at link and at run time.
+MPILIB := $(dir $(MPICXX))../lib
+$(OUT)/3dmhd_gpu: $(OUT)/obj/main_gpu.o $(OUT)/obj/gpu_hip.o $(OBJS) $(ROOT)/rast-3dmhd_amd/lib/librast3dmhd_hip.so
+	g++ -O2 -fopenmp -o $@ $(OUT)/obj/main_gpu.o $(OUT)/obj/gpu_hip.o $(OBJS) \
+	  $(MPILIB)/libmpicxx.so $(MPILIB)/libmpi.so -lm \
+	  -L$(ROOT)/rast-3dmhd_amd/lib -lrast3dmhd_hip \
+	  '-Wl,-rpath,/usr/lib/x86_64-linux-gnu:$(abspath $(MPILIB)):$$ORIGIN/../../rast-3dmhd_amd/lib'
