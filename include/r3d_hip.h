@@ -147,6 +147,20 @@ int r3d_hip_get_scalars(r3d_hip_ctx* ctx, r3d_hip_scalars* s);
  * from timc on the host between steps). */
 int r3d_hip_step(r3d_hip_ctx* ctx, int nsteps, int nstages);
 
+/* r3d_hip_step that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * kernels' scratch, the per-workgroup dt partials that r3d_hip_step keeps in
+ * the context's allocation, lives in a static device array of 1 MiB per GPU
+ * (131072 doubles; enough for about 26000 stage tiles, e.g. 504x504x2048
+ * takes 10066 x 5).  Larger grids get R3D_EINVAL.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (two contexts on
+ * different streams) race.  Results equal r3d_hip_step's bit for bit.
+ * Refused (R3D_ESTATE) while kernel timing is on, whose events are created
+ * per launch; with tc != 0 it synchronises the stream once per step, as
+ * r3d_hip_step does. */
+int r3d_hip_step_noalloc(r3d_hip_ctx* ctx, int nsteps, int nstages);
+
 /* Block until the context stream is idle. */
 int r3d_hip_sync(r3d_hip_ctx* ctx);
 

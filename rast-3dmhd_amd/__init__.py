@@ -87,6 +87,7 @@ def load():
         "r3d_hip_snapshot_begin": [vp],
         "r3d_hip_snapshot_rank": [vp, ip, ctypes.POINTER(dp), ctypes.POINTER(Scalars)],
         "r3d_hip_step": [vp, ip, ip],
+        "r3d_hip_step_noalloc": [vp, ip, ip],
         "r3d_hip_sync": [vp],
         "r3d_hip_set_timing": [vp, ip],
         "r3d_hip_kernel_stats": [vp, ip, ctypes.POINTER(ctypes.c_long),
@@ -234,6 +235,11 @@ class HipSim:
 
     def step(self, nsteps=1, nstages=3):
         _check(load().r3d_hip_step(self.h, nsteps, nstages))
+
+    def step_noalloc(self, nsteps=1, nstages=3):
+        """r3d_hip_step_noalloc: the same steps with no allocation, its scratch
+        in the library's static device array."""
+        _check(load().r3d_hip_step_noalloc(self.h, nsteps, nstages))
 
     def snapshot_begin(self):
         """Start an asynchronous snapshot of the current state (include/r3d_hip.h).

@@ -10,6 +10,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +21,16 @@ using namespace r3d;
 
 namespace {
 thread_local std::string g_err;
+
+// Scratch of r3d_hip_step_noalloc: the dt partials of its steps live in this
+// static device array instead of the context's allocation.  Each GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// the array's name is not a device address: the address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr size_t kNoallocScratchDoubles = size_t(1) << 17;  // 1 MiB
+__device__ __align__(256) double scratch[kNoallocScratchDoubles];
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;  // device ordinal -> address of scratch there
 }  // namespace
 
 int r3d::set_err(int code, const char* fmt, ...) {
@@ -299,6 +311,27 @@ int r3d::dev_alloc(r3d_hip_ctx* c, double** p, size_t n) {
     c->bytes += n * sizeof(double);
     return R3D_OK;
 }
+
+namespace {
+// Address of the static scratch on device dev, taken once per device.
+int noalloc_scratch(int dev, double** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return R3D_OK;
+    }
+    HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_err(R3D_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                       hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+    *out = static_cast<double*>(p);
+    return R3D_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -1116,6 +1149,35 @@ int r3d_hip_step(r3d_hip_ctx* c, int nsteps, int nstages) {
         if ((rc = step_one(c, nstages, nullptr))) return rc;
     if (c->timing) return drain_timing(c);
     return R3D_OK;
+}
+
+int r3d_hip_step_noalloc(r3d_hip_ctx* c, int nsteps, int nstages) {
+    if (!c) return set_err(R3D_EINVAL, "null context");
+    if (nstages < 1 || nstages > 3) return set_err(R3D_EINVAL, "nstages must be 1..3");
+    if (nsteps < 0 || (nstages < 3 && nsteps > 1))
+        return set_err(R3D_EINVAL, "a truncated step (nstages < 3) advances exactly one step");
+    if (c->timing)
+        return set_err(R3D_ESTATE, "step_noalloc: kernel timing is on (it creates events)");
+    int rc = check_ready(c);
+    if (rc) return rc;
+    // every partial slot a step reads is written earlier in the same step
+    // (stage-3 tiles, both boundary sides, or the dt pass), so the array's
+    // contents from an earlier call never reach a result
+    const size_t need = std::max<size_t>((size_t)c->nblocks + 2 * c->nbb, DT_BLOCKS) * NPART;
+    if (need > kNoallocScratchDoubles)
+        return set_err(R3D_EINVAL,
+                       "step_noalloc: this grid needs %zu doubles of dt partials; the static "
+                       "scratch holds %zu",
+                       need, kNoallocScratchDoubles);
+    double* sp = nullptr;
+    if ((rc = noalloc_scratch(c->cfg.device, &sp))) return rc;
+    // the kernels take the partials' address at launch: point the context at
+    // the static scratch for this call's launches only
+    double* const own = c->partial;
+    c->partial = sp;
+    for (int st = 0; st < nsteps && rc == R3D_OK; ++st) rc = step_one(c, nstages, nullptr);
+    c->partial = own;
+    return rc;
 }
 
 int r3d_hip_sync(r3d_hip_ctx* c) {
