@@ -77,6 +77,15 @@ typedef struct r3d_hip_cfg {
      * workgroup, 8, 16, 32, 64 or 128.  No reference counterpart: the
      * reference GPU launches one thread per cell (src/gpu.cpp). */
     int tile_rows, zchunk;
+    /* LREM (M. Rempel's stratification, golden/fortran/3dmhdsub.f:437-490,
+     * 817-851, 1316, 1534-1596, 2852-2952; the reference C++ refuses it,
+     * src/static_ic.cpp:27): lrem != 0 diffuses only the temperature
+     * perturbation about the horizontal mean, adds the mean profile's
+     * radiative heating, takes 1 + rkapa in the thermal time-step limit and
+     * the boundary values tu (constant top temperature), dztu and dztb (flux
+     * conditions) from the static set-up.  Bit-exact variant, NGRID=0, no RLAX. */
+    int lrem;
+    double tu, dztu, dztb;
 } r3d_hip_cfg;
 
 /* Global 1-D metrics.  x arrays have npx+2 entries, y arrays npy+2, z arrays

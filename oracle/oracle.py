@@ -39,6 +39,10 @@ def lib():
             getattr(L, fn).restype = ctypes.c_int
         L.o_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.o_step.restype = ctypes.c_int
+        L.o_lrem_profile.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 5
+        L.o_lrem_profile.restype = ctypes.c_int
+        L.o_lrem_background.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.o_lrem_background.restype = ctypes.c_int
         L.o_step_dt.argtypes = [ctypes.c_void_p]
         L.o_step_dt.restype = ctypes.c_int
         L.o_fluxes.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -82,6 +86,14 @@ def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def lrem_background(z):
+    """T, rho of the LREM background at depth z (oracle restatement)."""
+    out = (ctypes.c_double * 2)()
+    if lib().o_lrem_background(float(z), out) != 0:
+        raise OracleError(lib().o_last_error().decode())
+    return out[0], out[1]
+
+
 class OracleError(RuntimeError):
     pass
 
@@ -121,6 +133,14 @@ class Sim:
     def step(self, nsteps=1, nstages=3):
         for _ in range(nsteps):
             self._check(lib().o_step(self.h, nstages))
+
+    def lrem_profile(self):
+        """LREM STATIC profile: dict of T, R, RK, DRK (npz each) and TU, DZTU, DZTB."""
+        n = self.npz
+        a = [np.zeros(n) for _ in range(4)]
+        b = np.zeros(3)
+        self._check(lib().o_lrem_profile(self.h, *[_dptr(x) for x in a], _dptr(b)))
+        return dict(zip(["T", "R", "RK", "DRK"], a), TU=b[0], DZTU=b[1], DZTB=b[2])
 
     def step_dt(self):
         """STEP's time-step limits alone: uu, vv (LMAG), ww1..ww3, umach, dt."""

@@ -435,6 +435,151 @@ static int static_profile(const o_params* p, int npz, double* tp, double* rp) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* LREM stratification (M. Rempel): golden/fortran/3dmhdsub.f:437-490       */
+/* (STATIC), 4442-4650 (GETBACKGROUND, DERIVS1, ODEINT, RKQC, RK4) and      */
+/* golden/fortran/3dmhdset.f:154-170 (KAPPA).  Not in the reference C++,    */
+/* which refuses LREM (src/static_ic.cpp:27).  The Fortran is compiled      */
+/* without -fdefault-real-8 (golden/build_golden.sh:52-55), so its          */
+/* single-precision literals are widened floats here (w4).                  */
+/* ------------------------------------------------------------------------ */
+static double lrem_kappa(double z, double rho) {
+    double krad = rho;
+    krad = krad + (1.25 - krad) * exp(-(5.0 * z));
+    return krad;
+}
+
+static void lrem_derivs(double x, const double* yt, double* dyt) {
+    const double nad = w4(0.41f), os = 1.0;
+    const double t = yt[0], rho = yt[1] / yt[0];
+    const double nrad = w4(0.4f) / lrem_kappa(x, rho);
+    (void)t;
+    dyt[0] = nrad <= os * nad ? nrad : nad;
+    dyt[1] = yt[1] / yt[0];
+}
+
+static void lrem_rk4(const double* y, const double* dydx, double x, double h, double* yout) {
+    double yt[2], dyt[2], dym[2];
+    const double hh = h * 0.5, h6 = h / 6.0, xh = x + hh;
+    for (int i = 0; i < 2; ++i) yt[i] = y[i] + hh * dydx[i];
+    lrem_derivs(xh, yt, dyt);
+    for (int i = 0; i < 2; ++i) yt[i] = y[i] + hh * dyt[i];
+    lrem_derivs(xh, yt, dym);
+    for (int i = 0; i < 2; ++i) {
+        yt[i] = y[i] + h * dym[i];
+        dym[i] = dyt[i] + dym[i];
+    }
+    lrem_derivs(x + h, yt, dyt);
+    for (int i = 0; i < 2; ++i) yout[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
+}
+
+/* step doubling with error control (RKQC); returns -1 where the Fortran PAUSEs */
+static int lrem_rkqc(double* y, double* dydx, double* x, double htry, double eps,
+                     const double* yscal, double* hnext) {
+    const double fcor = 1.0 / 15.0, safety = w4(0.9f), errcon = 6.0e-4;
+    const double pgrow = w4(-0.20f), pshrnk = -0.25;
+    const double xsav = *x;
+    const double ysav[2] = {y[0], y[1]}, dysav[2] = {dydx[0], dydx[1]};
+    double ytemp[2], h = htry;
+    for (;;) {
+        const double hh = 0.5 * h;
+        lrem_rk4(ysav, dysav, xsav, hh, ytemp);
+        *x = xsav + hh;
+        lrem_derivs(*x, ytemp, dydx);
+        lrem_rk4(ytemp, dydx, *x, hh, y);
+        *x = xsav + h;
+        if (*x == xsav) return fail("LREM background: step size not significant");
+        lrem_rk4(ysav, dysav, xsav, h, ytemp);
+        double errmax = 0.0;
+        for (int i = 0; i < 2; ++i) {
+            ytemp[i] = y[i] - ytemp[i];
+            const double e = fabs(ytemp[i] / yscal[i]);
+            errmax = errmax > e ? errmax : e;
+        }
+        errmax = errmax / eps;
+        if (errmax > 1.0) {
+            h = safety * h * pow(errmax, pshrnk);
+            continue;
+        }
+        *hnext = errmax > errcon ? safety * h * pow(errmax, pgrow) : 4.0 * h;
+        break;
+    }
+    for (int i = 0; i < 2; ++i) y[i] = y[i] + ytemp[i] * fcor;
+    return 0;
+}
+
+/* ODEINT from x1 to x2 keeping only the end point (KMAX = 1) */
+static int lrem_odeint(const double* ystart, double x1, double x2, double eps, double h1,
+                       double hmin, double* yend) {
+    const double tiny = w4(1.0e-30f);
+    double x = x1, h = copysign(fabs(h1), x2 - x1);
+    double y[2] = {ystart[0], ystart[1]}, dydx[2] = {0.0, 0.0}, yscal[2];
+    if (x2 == x1) { /* the Fortran's first step would be 0 and PAUSE */
+        yend[0] = y[0];
+        yend[1] = y[1];
+        return 0;
+    }
+    for (int nstp = 0; nstp < 1000; ++nstp) {
+        lrem_derivs(x, y, dydx);
+        for (int i = 0; i < 2; ++i) yscal[i] = fabs(y[i]) + fabs(h * dydx[i]) + tiny;
+        if ((x + h - x2) * (x + h - x1) > 0.0) h = x2 - x;
+        double hnext = 0.0;
+        if (lrem_rkqc(y, dydx, &x, h, eps, yscal, &hnext)) return -1;
+        if ((x - x2) * (x2 - x1) >= 0.0) {
+            yend[0] = y[0];
+            yend[1] = y[1];
+            return 0;
+        }
+        if (fabs(hnext) < hmin) return fail("LREM background: step size smaller than minimum");
+        h = hnext;
+    }
+    return fail("LREM background: too many steps");
+}
+
+/* T(z), rho(z): integrate from z = 2 (T = 1, p = rho T = 1) to z */
+static int lrem_background(double z, double* t, double* rho) {
+    const double ys[2] = {1.0, 1.0};
+    double y[2] = {0.0, 0.0};
+    if (lrem_odeint(ys, 2.0, z, 1.0e-8, 1.0e-4, 1.0e-50, y)) return -1;
+    *t = y[0];
+    *rho = y[1] / y[0];
+    return 0;
+}
+
+/* STATIC's LREM profile on the npz planes: T, R, the conductivity RK scaled
+ * by 8.07*THETA*REPR and its derivative DRK (one-sided at the ends), and the
+ * boundary values TU = T(1), DZTU, DZTB (golden/fortran/3dmhdsub.f:437-490) */
+static int lrem_profile(const o_params* p, const o_derived* d, int npz, double* T, double* R,
+                        double* RK, double* DRK, double bnd[3]) {
+    for (int k = 0; k < npz; ++k) {
+        const double szz = w4((float)k / (float)(npz - 1));
+        double sz, sd, sd2;
+        mkgrid(p, szz, 3, &sz, &sd, &sd2);
+        double tz, rz;
+        if (lrem_background(sz, &tz, &rz)) return -1;
+        T[k] = tz;
+        R[k] = rz;
+        RK[k] = lrem_kappa(sz, rz);
+    }
+    const double c807 = w4(8.07f);
+    for (int k = 0; k < npz; ++k) RK[k] = RK[k] * c807 * p->theta * d->repr;
+    for (int k = 1; k < npz - 1; ++k) {
+        double sz, sd, sd2;
+        mkgrid(p, w4((float)k / (float)(npz - 1)), 3, &sz, &sd, &sd2);
+        DRK[k] = (RK[k + 1] - RK[k - 1]) * d->hz * sd;
+    }
+    double sz, sd, sd2;
+    mkgrid(p, 0.0, 3, &sz, &sd, &sd2);
+    DRK[0] = (-3.0 * RK[0] + 4.0 * RK[1] - RK[2]) * d->hz * sd;
+    mkgrid(p, 1.0, 3, &sz, &sd, &sd2);
+    DRK[npz - 1] = (3.0 * RK[npz - 1] - 4.0 * RK[npz - 2] + RK[npz - 3]) * d->hz * sd;
+    const double f43 = w4(4.0f / 3.0f), f13 = w4(1.0f / 3.0f);
+    bnd[0] = T[0];
+    bnd[1] = T[0] - f43 * T[1] + f13 * T[2];
+    bnd[2] = T[npz - 1] - f43 * T[npz - 2] + f13 * T[npz - 3];
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Simulation state.                                                        */
 /* ------------------------------------------------------------------------ */
 enum {
@@ -459,6 +604,7 @@ struct o_sim {
     int nranks;
     o_rank* r;
     double dt, timc, timt, timi, umach, tb;
+    double tu, dztu, dztb; /* LREM boundary values of STATIC (COMMON/SPECIALBOUND/) */
     double tfac, ttm_max; /* flux relaxation state kept across calls (SAVE / DATA) */
     int nit;
 };
@@ -568,5 +714,11 @@ void o_mkgrid(const o_sim* s, double sc, int ic, double out[3]) {
 int o_static_profile(const o_sim* s, double* t, double* r) {
     return static_profile(&s->p, s->p.npz, t, r);
 }
+
+int o_lrem_profile(const o_sim* s, double* t, double* r, double* rk, double* drk, double bnd[3]) {
+    return lrem_profile(&s->p, &s->d, s->p.npz, t, r, rk, drk, bnd);
+}
+
+int o_lrem_background(double z, double out[2]) { return lrem_background(z, &out[0], &out[1]); }
 
 #include "r3d_oracle_step.inc"

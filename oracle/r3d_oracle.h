@@ -77,6 +77,14 @@ int o_step(o_sim* s, int nstages);
  * STEP with LMAG (tests/test_lmag_pins.py). */
 int o_step_dt(o_sim* s);
 
+/* LREM stratification (golden/fortran/3dmhdsub.f:437-490, 4442-4650;
+ * golden/fortran/3dmhdset.f:154-170): per plane T, rho, the scaled
+ * conductivity RK and its derivative DRK, and bnd = {TU, DZTU, DZTB}; and the
+ * background T, rho at depth z alone.  Parity unpinned (no Fortran here):
+ * pinned by the ODE's own identities (tests/test_lrem.py). */
+int o_lrem_profile(const o_sim* s, double* t, double* r, double* rk, double* drk, double bnd[3]);
+int o_lrem_background(double z, double out[2]);
+
 /* Standalone pieces for invariant tests. */
 int o_fluxes(o_sim* s, int rank);      /* src/fluxes.cpp:44 */
 int o_bcon(o_sim* s, int rank);        /* src/bcon.cpp:12 */

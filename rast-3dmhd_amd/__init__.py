@@ -45,7 +45,8 @@ class Cfg(ctypes.Structure):
                  "tp", "tc", "hh", "xp", "yp", "dzf", "tb", "zp", "qfh", "dh", "dv", "zmax")] + \
                [("id", ctypes.c_int)] + \
                [(n, ctypes.c_double) for n in ("rlax", "tstart", "toff")] + \
-               [(n, ctypes.c_int) for n in ("tile_rows", "zchunk")]
+               [(n, ctypes.c_int) for n in ("tile_rows", "zchunk", "lrem")] + \
+               [(n, ctypes.c_double) for n in ("tu", "dztu", "dztb")]
 
 
 METRIC_NAMES = ["exx", "dxxdx", "d2xxdx2", "ddx", "wyy", "dyydy", "d2yydy2", "ddy",

@@ -910,6 +910,7 @@ int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** o
     if (cfg->itcon == 2) return set_err(R3D_EUNSUP, "adjoint: ITCON=2 heat loss is not supported");
     if (cfg->id != 0) return set_err(R3D_EUNSUP, "adjoint: density diffusion is not supported");
     if (cfg->rlax > 0.0) return set_err(R3D_EUNSUP, "adjoint: flux relaxation is not supported");
+    if (cfg->lrem) return set_err(R3D_EUNSUP, "adjoint: LREM is not supported");
     auto* a = new r3d_adj_ctx();
     a->K = checkpoint_every;
     a->nf = cfg->lmag ? 8 : 5;

@@ -61,12 +61,13 @@ __device__ __forceinline__ void bound_block(const StageArgs& a, const Consts& C,
                              C.c23 * M.dz_top *
                                  (C.theta - (C.theta - C.tpr) * M.e1[i] / C.hh * M.e2[j] / C.hh);
                         break;
-                    case TOP_CONST_T:
-                        tt = 1.0;
-                        if (C.half_t && i <= (g.npx + 2) / 2) tt = 1.0 - C.tp;
+                    case TOP_CONST_T:  // LREM: TU (golden/fortran/3dmhdsub.f:2852-2853)
+                        tt = C.lrem ? C.tu : 1.0;
+                        if (!C.lrem && C.half_t && i <= (g.npx + 2) / 2) tt = 1.0 - C.tp;
                         break;
-                    case TOP_CONST_FLUX:
-                        tt = C.c43 * tt1 - C.c13 * tt2 - C.c23 * M.dz_top * C.theta;
+                    case TOP_CONST_FLUX:  // LREM: + DZTU (:2876-2879)
+                        tt = C.lrem ? C.c43 * tt1 - C.c13 * tt2 + C.dztu
+                                    : C.c43 * tt1 - C.c13 * tt2 - C.c23 * M.dz_top * C.theta;
                         break;
                     default:
                         break;
@@ -74,6 +75,8 @@ __device__ __forceinline__ void bound_block(const StageArgs& a, const Consts& C,
             } else {
                 if (C.izcon == 0)
                     tt = C.tb;
+                else if (C.lrem)  // + DZTB (golden/fortran/3dmhdsub.f:2949-2952)
+                    tt = C.c43 * tt1 - C.c13 * tt2 + C.dztb;
                 else
                     tt = C.c43 * tt1 - C.c13 * tt2 + C.c23 * M.dz_bot * C.theta / M.rkapa[kb];
             }
@@ -109,9 +112,9 @@ __device__ __forceinline__ void bound_block(const StageArgs& a, const Consts& C,
             if (a.stage == 2 && a.want_dt) {
                 const double dd = dmin(dmin(M.ddx[i], M.ddy[j]), M.ddz[kb]);
                 if (C.lmag)
-                    dt_terms<true>(C, ru, rv, rw, ro, tt, bx, by, bz, dd, M.rkapa[kb], pm);
+                    dt_terms<true>(C, ru, rv, rw, ro, tt, bx, by, bz, dd, M.rkdt[kb], pm);
                 else
-                    dt_terms<false>(C, ru, rv, rw, ro, tt, 0.0, 0.0, 0.0, dd, M.rkapa[kb], pm);
+                    dt_terms<false>(C, ru, rv, rw, ro, tt, 0.0, 0.0, 0.0, dd, M.rkdt[kb], pm);
             }
         }
     }
@@ -135,10 +138,10 @@ __device__ __forceinline__ void dt_block(const Geo& g, const State& s, const Con
         const double dd = dmin(dmin(M.ddx[i], M.ddy[j]), M.ddz[k]);
         if (C.lmag)
             dt_terms<true>(C, s.q[QRU][n], s.q[QRV][n], s.q[QRW][n], s.q[QRO][n], s.q[QTT][n],
-                           s.q[QBX][n], s.q[QBY][n], s.q[QBZ][n], dd, M.rkapa[k], pm);
+                           s.q[QBX][n], s.q[QBY][n], s.q[QBZ][n], dd, M.rkdt[k], pm);
         else
             dt_terms<false>(C, s.q[QRU][n], s.q[QRV][n], s.q[QRW][n], s.q[QRO][n], s.q[QTT][n],
-                            0.0, 0.0, 0.0, dd, M.rkapa[k], pm);
+                            0.0, 0.0, 0.0, dd, M.rkdt[k], pm);
     }
     block_partials<DT_THREADS / 64>(pm, red, partial, bx);
 }

@@ -62,6 +62,11 @@ struct Consts {
     // (DH != 0, DV != 0); read by the DEN stage kernels only
     int den_h, den_v;
     int rlx;  // flux relaxation on: FT gains the per-plane RLAX*TFAC*CORRECT (DEN kernels)
+    // LREM (DEN kernels): FT diffuses TT - TTM(K) with unit conductivity and
+    // gains the mean profile's radiative heating (golden/fortran/3dmhdsub.f:1551-1596);
+    // BCON takes TU, DZTU, DZTB (:2852-2952)
+    int lrem;
+    double tu, dztu, dztb;
     int lrot, lmag, top_mode, half_t, izcon, bz_zero;
     int mag_bot;  // LMAG one-sided z stencils on the bottom wall (npez > 1,
                   // golden/fortran/3dmhdsub.f:1126, 1392, 1513 sit in an ELSE IF)
@@ -81,6 +86,12 @@ struct Metrics {
     // flux relaxation per global plane: RLAX*TFAC*CORRECT(K), rewritten
     // before every stage; and the plane coordinates
     const double *ftc, *zee;
+    // LREM per global plane, rewritten before every stage: TTM (horizontal
+    // mean of TT) and the radiative heating HRAD(K) of the mean profile
+    const double *ttm, *hrad;
+    // conductivity in the thermal time-step limit: rkapa, or 1 + rkapa with
+    // LREM (golden/fortran/3dmhdsub.f:845-851)
+    const double* rkdt;
     double dz_top, dz_bot;  // f4(1/(npz-1)) / dzzdz at the boundary planes
 };
 

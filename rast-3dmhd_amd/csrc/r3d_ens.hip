@@ -267,6 +267,8 @@ int r3d_ens_create(int nmembers, const r3d_hip_cfg* cfgs, r3d_ens_ctx** out) {
             return set_err(R3D_EUNSUP, "member %d: flux relaxation is not built in an ensemble", m);
         if (cfgs[m].itcon == 2)
             return set_err(R3D_EUNSUP, "member %d: ITCON=2 heat loss is not built in an ensemble", m);
+        if (cfgs[m].lrem)
+            return set_err(R3D_EUNSUP, "member %d: LREM is not built in an ensemble", m);
     }
     auto* e = new r3d_ens_ctx();
     e->device = cfgs[0].device;

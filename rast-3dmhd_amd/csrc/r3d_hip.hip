@@ -195,6 +195,20 @@ __global__ __launch_bounds__(DEN_THREADS) void k_hmean_final(const double* part,
     out[K] = acc;
 }
 
+// LREM radiative heating of the mean profile per plane K = 1..npz
+// (golden/fortran/3dmhdsub.f:1574-1580): HRAD = (d2 TTM/dz2 terms) RKAPA
+// + dTTM/dz DKAPA, in the Fortran's operation order.
+__global__ __launch_bounds__(DEN_THREADS) void k_lrem_hrad(const double* ttm, double* hrad,
+                                                           const Metrics M, int npz, double hz,
+                                                           double h2z) {
+    const int K = 1 + blockIdx.x * DEN_THREADS + threadIdx.x;
+    if (K > npz) return;
+    const double dz = M.dzz[K];
+    hrad[K] = ((ttm[K + 1] - 2.0 * ttm[K] + ttm[K - 1]) * h2z * dz * dz +
+               (ttm[K + 1] - ttm[K - 1]) * hz * M.d2zz[K]) * M.rkapa[K] +
+              (ttm[K + 1] - ttm[K - 1]) * hz * dz * M.dkapa[K];
+}
+
 // Flux relaxation, one thread (golden/fortran/3dmhdsub.f:1625-1771, the
 // ITC=0 / IZC=1 branch): the TFAC update from the bottom-wall TTM, then per z
 // rank (nrz planes, the rank's lower ghost plane being the next rank's first)
@@ -384,6 +398,10 @@ void fill_consts(r3d_hip_ctx* c) {
     C.den_h = f.id != 0 && f.dh != 0.0;
     C.den_v = f.id != 0 && f.dv != 0.0;
     C.rlx = f.rlax > 0.0;
+    C.lrem = f.lrem != 0;
+    C.tu = f.tu;
+    C.dztu = f.dztu;
+    C.dztb = f.dztb;
     C.gam[0] = f.gam1;
     C.gam[1] = f.gam2;
     C.gam[2] = f.gam3;
@@ -512,7 +530,7 @@ int launch_relax_prep(r3d_hip_ctx* c, int stage) {
     double* fconm = ttm + nkz;
     double* ftc = fconm + nkz;
     double* alpha = ftc + nkz;
-    double* st = c->den_buf + nkz * (f.npey + 8);
+    double* st = c->den_buf + nkz * (f.npey + 9);
     State s;
     for (int q = 0; q < NXF; ++q) s.q[q] = c->xs[c->cur][q];
     const double cnt = (double)(f.npx * f.npy);
@@ -536,14 +554,40 @@ int launch_relax_prep(r3d_hip_ctx* c, int stage) {
     return R3D_OK;
 }
 
+// LREM set-up of this stage's input: TTM = the horizontal mean of TT in the
+// reference's order, then HRAD per plane (golden/fortran/3dmhdsub.f:1540, 1574-1580).
+int launch_lrem_prep(r3d_hip_ctx* c) {
+    const r3d_hip_cfg& f = c->cfg;
+    const int nk = f.npz + 2;
+    const size_t nkz = (size_t)nk;
+    double* part = c->den_buf;
+    double* ttm = c->den_buf + nkz * (f.npey + 4);
+    double* hrad = c->den_buf + nkz * (f.npey + 8);
+    const dim3 gp((nk * f.npey + DEN_PARTIAL_THREADS - 1) / DEN_PARTIAL_THREADS);
+    const dim3 gf((nk + DEN_THREADS - 1) / DEN_THREADS);
+    hipLaunchKernelGGL(k_den_partial, gp, dim3(DEN_PARTIAL_THREADS), 0, c->stream, c->g,
+                       (const double*)c->xs[c->cur][QTT], part, f.npey,
+                       (double)(f.npx * f.npy));
+    hipLaunchKernelGGL(k_hmean_final, gf, dim3(DEN_THREADS), 0, c->stream, (const double*)part, ttm,
+                       nk, f.npey);
+    hipLaunchKernelGGL(k_lrem_hrad, gf, dim3(DEN_THREADS), 0, c->stream, (const double*)ttm, hrad,
+                       c->M, f.npz, f.hz, f.h2z);
+    HIPCHK(hipGetLastError());
+    return R3D_OK;
+}
+
 int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     const StageArgs a = stage_args(c, stage, zmode, want_dt);
     hipError_t fma_err = hipSuccess;
     int rc;
-    if (c->cfg.id != 0 && (rc = launch_density_prep(c))) return rc;
+    // RO's mirrored outer planes with density diffusion or LREM
+    // (golden/fortran/3dmhdsub.f:1316); then TTM and HRAD with LREM
+    if ((c->cfg.id != 0 || c->cfg.lrem) && (rc = launch_density_prep(c))) return rc;
     if (c->cfg.rlax > 0.0 && (rc = launch_relax_prep(c, stage))) return rc;
+    if (c->cfg.lrem && (rc = launch_lrem_prep(c))) return rc;
     rc = timed(c, 0, [&] {
-        const bool ht = c->cfg.itcon == 2, dn = c->cfg.id != 0 || c->cfg.rlax > 0.0;
+        const bool ht = c->cfg.itcon == 2,
+                   dn = c->cfg.id != 0 || c->cfg.rlax > 0.0 || c->cfg.lrem;
         if (c->cfg.variant == 1) {
             fma_err = launch_stage_fma(a, c->C, c->M, c->ty, c->cfg.lmag != 0, c->grid,
                                        c->stream);
@@ -698,7 +742,11 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     }
     if (f.variant != 0 && f.variant != 1)
         return set_err(R3D_EUNSUP, "variant %d: 0 (bit-exact) and 1 (FMA) are built", f.variant);
-    const bool feature = f.itcon == 2 || f.id != 0 || f.rlax > 0.0;
+    if (f.lrem && f.variant != 0)
+        return set_err(R3D_EUNSUP, "LREM is built in the bit-exact variant only");
+    if (f.lrem && f.rlax > 0.0)
+        return set_err(R3D_EUNSUP, "flux relaxation with LREM (FTOT = 8.07*0.4*THETA) is not built");
+    const bool feature = f.itcon == 2 || f.id != 0 || f.rlax > 0.0 || f.lrem;
     if (f.tile_rows != 0) {
         const bool ok = f.lmag ? f.tile_rows == TILE_Y_MAG
                                : feature ? f.tile_rows == 8 : (f.tile_rows == 8 || f.tile_rows == TILE_Y);
@@ -773,10 +821,12 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
             return fail(set_err(R3D_EHIP, "hipMemcpy scalars failed"));
     }
     const size_t nmet = 3 * (size_t)(f.npx + 2) + 3 * (size_t)(f.npy + 2) + 5 * (size_t)(f.npz + 2) +
-                        2 * ((size_t)(f.npx + 2) + (size_t)(f.npy + 2)) + 2 * (size_t)(f.npz + 2);
+                        2 * ((size_t)(f.npx + 2) + (size_t)(f.npy + 2)) + 3 * (size_t)(f.npz + 2);
     if ((rc = dev_alloc(c, &c->metric_buf, nmet))) return fail(rc);
-    if (f.id != 0 || f.rlax > 0.0) {
-        const size_t nk = (size_t)f.npz + 2, n = nk * ((size_t)f.npey + 8) + 2;
+    if (f.id != 0 || f.rlax > 0.0 || f.lrem) {
+        // rows of npz+2: npey partial means, ROM, DH*ORE*WWZ, DV*ORE*WWZ, the ID=2
+        // profile, TTM, FCONM, RLAX*TFAC*CORRECT, ALPHA, HRAD; then TFAC, TTM_MAX
+        const size_t nk = (size_t)f.npz + 2, n = nk * ((size_t)f.npey + 9) + 2;
         if ((rc = dev_alloc(c, &c->den_buf, n))) return fail(rc);
         // zero rows the run does not compute; TFAC = 1, TTM_MAX = -1e99
         // (DATA statements, golden/fortran/3dmhdsub.f:1258-1259)
@@ -790,6 +840,8 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
         c->M.dth = c->M.rom + nk;
         c->M.dtv = c->M.dth + nk;
         c->M.ftc = c->M.rom + 6 * nk;
+        c->M.ttm = c->M.rom + 4 * nk;
+        c->M.hrad = c->M.rom + 8 * nk;
     }
     if (f.rlax > 0.0 && (rc = dev_alloc(c, &c->ww2_buf, field_elems(c->g)))) return fail(rc);
     const size_t stage_elems = (size_t)std::max<long>((long)c->nx * c->ny * c->nz,
@@ -892,6 +944,12 @@ int r3d_hip_set_metrics(r3d_hip_ctx* c, const r3d_hip_metrics* m) {
     M.hgy = put(hg.data() + X, Y);
     M.hgz = put(hg.data() + X + Y, Z);
     M.zee = put(zee.data(), Z);
+    // the thermal time-step limit's conductivity: 1 + RKAPA with LREM
+    // (golden/fortran/3dmhdsub.f:845-851), RKAPA otherwise (src/step.cpp:197)
+    std::vector<double> rkdt(m->rkapa, m->rkapa + Z);
+    if (f.lrem)
+        for (int k = 0; k < Z; ++k) rkdt[k] = 1.0 + m->rkapa[k];
+    M.rkdt = put(rkdt.data(), Z);
     M.dz_top = f.dzf / m->dzzdz[1];
     M.dz_bot = f.dzf / m->dzzdz[f.npz];
     // ID=2 density diffusion profile WWZ = EXP(CLN*ZEE**2) + EXP(CLN*(ZEE-ZMAX)**2),

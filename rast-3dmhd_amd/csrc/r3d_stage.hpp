@@ -221,14 +221,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // uniform per plane, and fetching them inside the z loop would put a
     // global-memory round trip on every plane.
     constexpr int KCMAX = 128;
-    __shared__ double zm[5][KCMAX];  // dzz, d2zz, ddz, rkapa, dkapa
+    // dzz, d2zz, ddz, the conductivity (rkapa; 1 + rkapa with LREM, whose
+    // diffusion does not read it), dkapa
+    __shared__ double zm[5][KCMAX];
     // DEN / HEAT per-plane values, staged with the metrics for the same
     // reason: ROM at k-1..k+1 (row 0, offset by one plane), DH*ORE*WWZ,
     // DV*ORE*WWZ, RLAX*TFAC*CORRECT, then the heat loss z Gaussian; and the heat loss x / y
     // Gaussians of the tile's columns and rows, so that they hold no
     // registers across the z loop
-    constexpr int ZX_DEN = 0, ZX_HEAT = DEN ? 4 : 0;
-    constexpr int NZX = (DEN ? 4 : 0) + (HEAT ? 1 : 0);
+    // and with LREM TTM at k-1..k+1 (row ZX_DEN + 4, offset by one plane) and HRAD
+    constexpr int ZX_DEN = 0, ZX_HEAT = DEN ? 6 : 0;
+    constexpr int NZX = (DEN ? 6 : 0) + (HEAT ? 1 : 0);
     __shared__ double zx[NZX > 0 ? NZX : 1][KCMAX + 2];
     __shared__ double hgx_s[HEAT ? 66 : 1], hgy_s[HEAT ? ROWS : 1];
     // Stage 3: the column's min(ddx, ddy) and the lane's running dt extrema
@@ -293,18 +296,22 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     for (int q = threadIdx.x; q < 5 * KCMAX; q += blockDim.x) {
         const int f = q / KCMAX, kk = k0 + q % KCMAX;
         if (kk <= k1) {
-            const double* src = f == 0 ? M.dzz : f == 1 ? M.d2zz : f == 2 ? M.ddz : f == 3 ? M.rkapa : M.dkapa;
+            const double* src = f == 0 ? M.dzz : f == 1 ? M.d2zz : f == 2 ? M.ddz : f == 3 ? M.rkdt : M.dkapa;
             zm[f][kk - k0] = src[kk];
         }
     }
     if constexpr (DEN) {
         for (int q = threadIdx.x; q < KCMAX + 2; q += blockDim.x) {
             const int kk = k0 - 1 + q;  // planes k0-1 .. k1+1 (within 0 .. npz+1)
-            if (kk <= k1 + 1) zx[ZX_DEN][q] = M.rom[kk];
+            if (kk <= k1 + 1) {
+                zx[ZX_DEN][q] = M.rom[kk];
+                zx[ZX_DEN + 4][q] = M.ttm[kk];
+            }
             if (q < KCMAX && k0 + q <= k1) {
                 zx[ZX_DEN + 1][q] = M.dth[k0 + q];
                 zx[ZX_DEN + 2][q] = M.dtv[k0 + q];
                 zx[ZX_DEN + 3][q] = M.ftc[k0 + q];
+                zx[ZX_DEN + 5][q] = M.hrad[k0 + q];
             }
         }
     }
@@ -324,7 +331,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     __syncthreads();
 
     const double cg = C.gam[STAGE] * dt;
-    const double cz = (STAGE < 2) ? C.zeta[STAGE] * dt : 0.0;
+    const double cz = (STAGE < 2) ? C.zeta[STAGE < 2 ? STAGE : 0] * dt : 0.0;
 
     // x/y metric factors of this column (src/fluxes.cpp:90, 250-254)
     const double dxx = M.dxx[li], d2xx = M.d2xx[li];
@@ -572,12 +579,31 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 double ft = -1.0 * cc.u * (ttE - ttW) * hxd * dxx;
                 ft = ft - cc.v * (ttN - ttS) * tmpy;
                 ft = ft - cc.w * (ttU - ttD) * tmpz;
-                ft = ft + ((ttE - ttW) * hxd * d2xx + (ttE - t2c + ttW) * C.h2x * dxx * dxx) *
-                              cc.ri * C.tk * rk;
-                ft = ft + ((ttN - ttS) * tmpy1 + (ttN - t2c + ttS) * tmpy2) * cc.ri * C.tk * rk;
-                ft = ft + ((ttU - ttD) * tmpz1 + (ttU - t2c + ttD) * tmpz2) * cc.ri * C.tk * rk;
-                ft = ft + (ttU - ttD) * C.hz * dzz * cc.ri * C.tk * dk;
-                if constexpr (DEN) {  // flux relaxation, golden/fortran/3dmhdsub.f:1765-1771
+                bool lrem_diff = false;
+                if constexpr (DEN) lrem_diff = C.lrem;
+                if (!lrem_diff) {
+                    ft = ft + ((ttE - ttW) * hxd * d2xx + (ttE - t2c + ttW) * C.h2x * dxx * dxx) *
+                                  cc.ri * C.tk * rk;
+                    ft = ft + ((ttN - ttS) * tmpy1 + (ttN - t2c + ttS) * tmpy2) * cc.ri * C.tk * rk;
+                    ft = ft + ((ttU - ttD) * tmpz1 + (ttU - t2c + ttD) * tmpz2) * cc.ri * C.tk * rk;
+                    ft = ft + (ttU - ttD) * C.hz * dzz * cc.ri * C.tk * dk;
+                }
+                if constexpr (DEN) {
+                    if (C.lrem) {
+                        // LREM: diffusion of WW1 = TT - TTM(K) with unit conductivity,
+                        // then RO*TMP*HRAD(K) (golden/fortran/3dmhdsub.f:1551-1586)
+                        const double tm = zx[ZX_DEN + 4][kl + 1];
+                        const double tmU = zx[ZX_DEN + 4][kl + 2], tmD = zx[ZX_DEN + 4][kl];
+                        const double w0 = tt0 - tm, wE = ttE - tm, wW = ttW - tm;
+                        const double wN = ttN - tm, wS = ttS - tm;
+                        const double wU = ttU - tmU, wD = ttD - tmD;
+                        ft = ft + ((wE - wW) * hxd * d2xx + (wE - 2.0 * w0 + wW) * C.h2x * dxx * dxx) *
+                                      cc.ri * C.tk;
+                        ft = ft + ((wN - wS) * tmpy1 + (wN - 2.0 * w0 + wS) * tmpy2) * cc.ri * C.tk;
+                        ft = ft + ((wU - wD) * tmpz1 + (wU - 2.0 * w0 + wD) * tmpz2) * cc.ri * C.tk;
+                        ft = ft + cc.ri * C.tk * zx[ZX_DEN + 5][kl];
+                    }
+                    // flux relaxation, golden/fortran/3dmhdsub.f:1765-1771
                     if (C.rlx) ft = ft + zx[ZX_DEN + 3][kl];
                 }
                 if constexpr (HEAT) {

@@ -412,6 +412,144 @@ class Profile {
     }
 };
 
+// ------------------------------------------------------- LREM profile ----
+// M. Rempel's stratification (golden/fortran/3dmhdsub.f:437-490, STATIC;
+// 4442-4650, GETBACKGROUND / DERIVS1 / ODEINT / RKQC / RK4;
+// golden/fortran/3dmhdset.f:154-170, KAPPA).  The reference C++ refuses LREM
+// (src/static_ic.cpp:27).  The Fortran is built without -fdefault-real-8, so
+// its single-precision literals are widened floats here.  T(z) and
+// p = rho*T are integrated from z = 2 (T = p = 1) to each plane's depth by
+// Runge-Kutta with step doubling (tolerance 1e-8): dT/dz = min-like switch
+// between the radiative gradient 0.4/kappa and the adiabatic 0.41,
+// dp/dz = p/T, kappa = rho + (1.25 - rho) exp(-5 z).
+class LremProfile {
+   public:
+    LremProfile(const Params& p, const Derived& d) : p_(p), d_(d) {}
+
+    // T, R, the conductivity RK = kappa*8.07*theta*repr and its derivative
+    // DRK per plane; bnd = {TU, DZTU, DZTB}
+    int build(int npz, std::vector<double>& T, std::vector<double>& R, std::vector<double>& RK,
+              std::vector<double>& DRK, double bnd[3]) const {
+        T.assign(npz, 0.0);
+        R.assign(npz, 0.0);
+        RK.assign(npz, 0.0);
+        DRK.assign(npz, 0.0);
+        for (int k = 0; k < npz; ++k) {
+            const MapOut m = grid_map(p_, single((float)k / (float)(npz - 1)), 3);
+            double t = 0.0, rho = 0.0;
+            if (background(m.x, t, rho)) return -1;
+            T[k] = t;
+            R[k] = rho;
+            RK[k] = kappa(m.x, rho);
+        }
+        const double c807 = single(8.07f);
+        for (int k = 0; k < npz; ++k) RK[k] = RK[k] * c807 * p_.theta * d_.repr;
+        for (int k = 1; k < npz - 1; ++k) {
+            const MapOut m = grid_map(p_, single((float)k / (float)(npz - 1)), 3);
+            DRK[k] = (RK[k + 1] - RK[k - 1]) * d_.hz * m.dsdx;
+        }
+        const MapOut m0 = grid_map(p_, 0.0, 3), m1 = grid_map(p_, 1.0, 3);
+        DRK[0] = (-3.0 * RK[0] + 4.0 * RK[1] - RK[2]) * d_.hz * m0.dsdx;
+        DRK[npz - 1] = (3.0 * RK[npz - 1] - 4.0 * RK[npz - 2] + RK[npz - 3]) * d_.hz * m1.dsdx;
+        const double f43 = single(4.0f / 3.0f), f13 = single(1.0f / 3.0f);
+        bnd[0] = T[0];
+        bnd[1] = T[0] - f43 * T[1] + f13 * T[2];
+        bnd[2] = T[npz - 1] - f43 * T[npz - 2] + f13 * T[npz - 3];
+        return 0;
+    }
+
+   private:
+    const Params& p_;
+    const Derived& d_;
+
+    static double kappa(double z, double rho) {
+        double krad = rho;
+        krad = krad + (1.25 - krad) * std::exp(-(5.0 * z));
+        return krad;
+    }
+
+    static void rhs(double x, const double* y, double* dy) {
+        const double nad = single(0.41f), os = 1.0;
+        const double rho = y[1] / y[0];
+        const double nrad = single(0.4f) / kappa(x, rho);
+        dy[0] = nrad <= os * nad ? nrad : nad;
+        dy[1] = y[1] / y[0];
+    }
+
+    static void rk4(const double* y, const double* dy, double x, double h, double* out) {
+        double yt[2], dyt[2], dym[2];
+        const double hh = h * 0.5, h6 = h / 6.0, xh = x + hh;
+        for (int i = 0; i < 2; ++i) yt[i] = y[i] + hh * dy[i];
+        rhs(xh, yt, dyt);
+        for (int i = 0; i < 2; ++i) yt[i] = y[i] + hh * dyt[i];
+        rhs(xh, yt, dym);
+        for (int i = 0; i < 2; ++i) {
+            yt[i] = y[i] + h * dym[i];
+            dym[i] = dyt[i] + dym[i];
+        }
+        rhs(x + h, yt, dyt);
+        for (int i = 0; i < 2; ++i) out[i] = y[i] + h6 * (dy[i] + dyt[i] + 2.0 * dym[i]);
+    }
+
+    // one accepted step with error control by step doubling
+    static int step(double* y, double* dy, double& x, double htry, double eps,
+                    const double* yscal, double& hnext) {
+        const double fcor = 1.0 / 15.0, safety = single(0.9f), errcon = 6.0e-4;
+        const double pgrow = single(-0.20f), pshrnk = -0.25;
+        const double x0 = x, y0[2] = {y[0], y[1]}, d0[2] = {dy[0], dy[1]};
+        double err[2], h = htry;
+        for (;;) {
+            const double hh = 0.5 * h;
+            rk4(y0, d0, x0, hh, err);
+            x = x0 + hh;
+            rhs(x, err, dy);
+            rk4(err, dy, x, hh, y);
+            x = x0 + h;
+            if (x == x0) return fail("LREM background: step size not significant");
+            rk4(y0, d0, x0, h, err);
+            double emax = 0.0;
+            for (int i = 0; i < 2; ++i) {
+                err[i] = y[i] - err[i];
+                const double e = std::fabs(err[i] / yscal[i]);
+                emax = emax > e ? emax : e;
+            }
+            emax = emax / eps;
+            if (emax > 1.0) {
+                h = safety * h * std::pow(emax, pshrnk);
+                continue;
+            }
+            hnext = emax > errcon ? safety * h * std::pow(emax, pgrow) : 4.0 * h;
+            break;
+        }
+        for (int i = 0; i < 2; ++i) y[i] = y[i] + err[i] * fcor;
+        return 0;
+    }
+
+    static int background(double z, double& t, double& rho) {
+        const double x1 = 2.0, x2 = z, eps = 1.0e-8, hmin = 1.0e-50, tiny = single(1.0e-30f);
+        double x = x1, h = std::copysign(1.0e-4, x2 - x1);
+        double y[2] = {1.0, 1.0}, dy[2] = {0.0, 0.0}, ys[2];
+        if (x2 != x1) {  // a zero-length integration: the start values (the Fortran pauses)
+            int n = 0;
+            for (;; ++n) {
+                if (n == 1000) return fail("LREM background: too many steps");
+                rhs(x, y, dy);
+                for (int i = 0; i < 2; ++i) ys[i] = std::fabs(y[i]) + std::fabs(h * dy[i]) + tiny;
+                if ((x + h - x2) * (x + h - x1) > 0.0) h = x2 - x;
+                double hnext = 0.0;
+                if (step(y, dy, x, h, eps, ys, hnext)) return -1;
+                if ((x - x2) * (x2 - x1) >= 0.0) break;
+                if (std::fabs(hnext) < hmin)
+                    return fail("LREM background: step size smaller than minimum");
+                h = hnext;
+            }
+        }
+        t = y[0];
+        rho = y[1] / y[0];
+        return 0;
+    }
+};
+
 }  // namespace
 
 // Storage of the large global fields: not zero-initialised on resize (every
@@ -460,6 +598,7 @@ struct r3d_host {
     Params p;
     Derived d;
     double tb = 1.0;
+    double tu = 1.0, dztu = 0.0, dztb = 0.0;  // LREM boundary values of STATIC
     std::vector<double> m1[14];  // global metrics, gstate order
     BigVec f[8];                 // global initial primaries
     r3d_hip_scalars sc{};        // starting scalars (from the dumps on a restart)
@@ -511,6 +650,34 @@ void build_metrics(r3d_host* h) {
     }
 }
 
+// The static profile T, rho per plane: the polytrope, or with LREM M.
+// Rempel's stratification, which also sets the conductivity RKAPA, DKAPA of
+// every plane (golden/fortran/3dmhd.f:267-269; ghost planes take the end
+// values, golden/fortran/3dmhdsub.f:530-560) and the boundary values TU,
+// DZTU, DZTB.  tb = T at the bottom either way.
+int static_profile(r3d_host* h, std::vector<double>& T, std::vector<double>& R) {
+    const Params& p = h->p;
+    if (!p.lrem) {
+        Profile prof(p);
+        if (prof.build(p.npz, T, R)) return -1;
+    } else {
+        std::vector<double> RK, DRK;
+        double bnd[3];
+        LremProfile prof(p, h->d);
+        if (prof.build(p.npz, T, R, RK, DRK, bnd)) return -1;
+        for (int k = 0; k < p.npz + 2; ++k) {
+            const int src = k == 0 ? 0 : k <= p.npz ? k - 1 : p.npz - 1;
+            h->m1[12][k] = RK[src];
+            h->m1[13][k] = DRK[src];
+        }
+        h->tu = bnd[0];
+        h->dztu = bnd[1];
+        h->dztb = bnd[2];
+    }
+    h->tb = T[p.npz - 1];
+    return 0;
+}
+
 // Global initial state of STATIC (reference src/static_ic.cpp:24-161) after
 // the first halo exchange: uniform planes of the profile, the optional
 // magnetic layer, the ran2 perturbation in global (k, j, i) order, then the
@@ -522,9 +689,7 @@ int build_state(r3d_host* h) {
     const int nf = p.lmag ? 8 : 5;  // B fields only exist with LMAG
     for (int q = 0; q < nf; ++q) h->f[q].resize(n);  // every cell is written below
     std::vector<double> T, R;
-    Profile prof(p);
-    if (prof.build(p.npz, T, R)) return -1;
-    h->tb = T[p.npz - 1];
+    if (static_profile(h, T, R)) return -1;
     const double rb = R[p.npz - 1];
     double* tt = h->f[R3D_TT].data();
     double* ro = h->f[R3D_RO].data();
@@ -673,9 +838,7 @@ int restart_state(r3d_host* h) {
             std::fill(h->f[q].data() + ka * plane, h->f[q].data() + kb * plane, 0.0);
     });
     std::vector<double> T, R;
-    Profile prof(p);
-    if (prof.build(p.npz, T, R)) return -1;
-    h->tb = T[p.npz - 1];
+    if (static_profile(h, T, R)) return -1;  // tb (and with LREM the conductivity)
     const int nry = p.npy / p.npey, nrz = p.npz / p.npez;
     const int nx = X, ny = nry + 2, nz = nrz + 2;
     const size_t cnt = (size_t)nx * ny * nz;
@@ -720,7 +883,13 @@ int validate(const Params& p) {
     if (p.nstart < 0) return fail("NSTART must be >= 0");
     // the reference's feature guards (src/fluxes.cpp:46-51, src/static_ic.cpp:26-29)
     if (p.lshr) return fail("unsupported feature enabled: LSHR");
-    if (p.lrem) return fail("unsupported feature enabled: LREM");
+    // LREM (golden/fortran/3dmhdsub.f:437-490, 817-851, 1316, 1534-1596,
+    // 2852-2952) is built; its mean temperature TTM uses the reference's
+    // horizontal_mean, NGRID=0 only, and its flux relaxation variant is not
+    if (p.lrem && p.ngrid != 0)
+        return fail("LREM needs NGRID=0 (horizontal_mean: only NGRID=0 is supported)");
+    if (p.lrem && p.rlax > 0.0)
+        return fail("flux relaxation with LREM (FTOT = 8.07*0.4*THETA) is not built");
     // flux relaxation (golden/fortran/3dmhdsub.f:1625-1771) is built for the
     // ITC=0 / IZC=1 branch; in the ITC=1 / IZC=0 branch the last z rank's
     // never-assigned FRADM(NZ) reaches every rank's correction (:1697-1760),
@@ -850,6 +1019,10 @@ int r3d_host_cfg(const r3d_host* h, r3d_hip_cfg* o) {
     o->rlax = p.rlax;
     o->tstart = p.tstart;
     o->toff = p.toff;
+    o->lrem = p.lrem;
+    o->tu = h->tu;
+    o->dztu = h->dztu;
+    o->dztb = h->dztb;
     return 0;
 }
 

@@ -99,10 +99,11 @@ def test_density_diffusion_refused_where_not_built():
 
 @pytest.mark.parametrize("extra", [DEN, ["--itcon", "2", "--tp", "1.5", "--zp", "27", "--hh", "6"],
                                    ["--ngrid", "0", "--itcon", "0", "--izcon", "1", "--rlax", "0.5",
-                                    "--toff", "0.5"]],
-                         ids=["density", "heat", "relax"])
+                                    "--toff", "0.5"],
+                                   ["--ngrid", "0", "--lrem", "1", "--zmax", "4"]],
+                         ids=["density", "heat", "relax", "lrem"])
 def test_extra_physics_refused_by_ensemble_and_adjoint(extra):
-    """The three extra physics branches run in the single-run bit-exact path only."""
+    """The extra physics branches run in the single-run bit-exact path only."""
     args = ["--npx", "24", "--npy", "16", "--npz", "24", "--npey", "1", "--npez", "2"] + extra
     run = pkg.Run(args)
     with pytest.raises(pkg.HipError, match="not built in an ensemble"):

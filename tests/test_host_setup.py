@@ -21,6 +21,12 @@ CASES = [
      "--sigma", "3", "--polys", "2", "--seed", "-1234"],
     ["--npx", "16", "--npy", "32", "--npz", "64", "--npey", "2", "--npez", "4", "--lmag", "1",
      "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2", "--bzp", "10"],
+    # LREM stratification (its conductivity replaces the PZP profile)
+    ["--npx", "16", "--npy", "16", "--npz", "32", "--npey", "2", "--npez", "2", "--lrem", "1",
+     "--ngrid", "0", "--zmax", "4", "--pzp", "2"],
+    ["--npx", "16", "--npy", "24", "--npz", "48", "--npey", "3", "--npez", "3", "--lrem", "1",
+     "--ngrid", "0", "--zmax", "3", "--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1",
+     "--bfh", "0.5", "--bzp", "1.5", "--theta", "0.3"],
 ]
 
 
@@ -50,6 +56,11 @@ def test_initial_state_matches_oracle(args):
                 want, got = want[:-1], got[:-1]
             assert np.array_equal(got, want), (r, name)
     assert run.tb() == sim.scalars()["tb"]
+    if "--lrem" in args:
+        pr = sim.lrem_profile()
+        cfg = run.cfg()
+        assert cfg.lrem == 1
+        assert (cfg.tu, cfg.dztu, cfg.dztb) == (pr["TU"], pr["DZTU"], pr["DZTB"])
 
 
 def test_cfg_constants_match_oracle():
