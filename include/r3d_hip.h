@@ -86,6 +86,15 @@ typedef struct r3d_hip_cfg {
      * conditions) from the static set-up.  Bit-exact variant, NGRID=0, no RLAX. */
     int lrem;
     double tu, dztu, dztb;
+    /* Time-step screen of the last stage (results do not depend on it; no
+     * reference counterpart).  A cell's exact time-step limits are evaluated
+     * only where a division-free bound says one of them may come within
+     * this relative margin of the previous step's extrema; the step then
+     * proves that the extrema it found are the exact ones, or redoes them
+     * with an exact pass over every cell.  0 = 0.05; a large value (1e9)
+     * evaluates every cell; a value in (-1, 0) puts the thresholds inside
+     * the previous extrema, so that every step takes the exact pass. */
+    double dt_screen;
 } r3d_hip_cfg;
 
 /* Global 1-D metrics.  x arrays have npx+2 entries, y arrays npy+2, z arrays

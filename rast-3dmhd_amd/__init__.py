@@ -46,7 +46,7 @@ class Cfg(ctypes.Structure):
                [("id", ctypes.c_int)] + \
                [(n, ctypes.c_double) for n in ("rlax", "tstart", "toff")] + \
                [(n, ctypes.c_int) for n in ("tile_rows", "zchunk", "lrem")] + \
-               [(n, ctypes.c_double) for n in ("tu", "dztu", "dztb")]
+               [(n, ctypes.c_double) for n in ("tu", "dztu", "dztb", "dt_screen")]
 
 
 METRIC_NAMES = ["exx", "dxxdx", "d2xxdx2", "ddx", "wyy", "dyydy", "d2yydy2", "ddy",
@@ -664,14 +664,16 @@ class Run:
         t.start()
         return t
 
-    def make_sim(self, variant=0, tile_rows=0, zchunk=0):
+    def make_sim(self, variant=0, tile_rows=0, zchunk=0, dt_screen=0.0):
         """Create the device context and upload this run's metrics, state and
         starting scalars.  tile_rows / zchunk override the stage-kernel tiling
-        (0 = the library's choice; include/r3d_hip.h)."""
+        and dt_screen the margin of the last stage's time-step screen (0 = the
+        library's choice; include/r3d_hip.h)."""
         cfg = self.cfg()
         cfg.variant = variant
         cfg.tile_rows = tile_rows
         cfg.zchunk = zchunk
+        cfg.dt_screen = dt_screen
         sim = HipSim(cfg)
         sim.set_metrics(self.metrics())
         sim.upload_global(self.state(copy=False))

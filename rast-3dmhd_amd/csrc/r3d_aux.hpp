@@ -175,8 +175,15 @@ __device__ __forceinline__ void halo_thread(const Geo& g, const State& x, int nf
 // umach_next (dt = sff * min of the limits, src/step.cpp:224-229; umach =
 // sqrt(max), src/step.cpp:141).  advance: the end-of-step counters
 // (src/step.cpp:254-256).
+// screened: the partials of stage 3 were screened against sc->lim_prev /
+// um_prev; the extrema found are exact only if they reach the screen's
+// thresholds (the limit at most lim_prev*scr_up, ogamma*UU/T at least
+// um_prev*scr_dn): a cell with a smaller limit or a larger ogamma*UU/T would
+// have passed the screen.  Otherwise dt_redo is set and dt_next is left to
+// the exact pass.
 __device__ __forceinline__ void finish_block(DevScal* sc, const double* partial, int npart,
-                                             const Consts& C, int reduce, int advance) {
+                                             const Consts& C, int reduce, int advance,
+                                             int screened = 0) {
     __shared__ double red[16][NPART];
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
     if (reduce)
@@ -204,10 +211,18 @@ __device__ __forceinline__ void finish_block(DevScal* sc, const double* partial,
                 o[0] = dmax(o[0], red[w][0]);
                 for (int q = 1; q < NPART; ++q) o[q] = dmin(o[q], red[w][q]);
             }
-            sc->umach_next = sqrt(o[0]);
             double m = dmin(dmin(o[1], o[2]), o[3]);
             if (C.lmag) m = dmin(m, o[4]);
-            sc->dt_next = C.dtfix > 0.0 ? C.dtfix : C.sff * m;
+            // the thresholds exactly as the stage kernel formed them
+            const bool proven = !screened || (m <= sc->lim_prev * C.scr_up &&
+                                              o[0] >= sc->um_prev * C.scr_dn);
+            sc->dt_redo = proven ? 0 : 1;
+            if (proven) {
+                sc->umach_next = sqrt(o[0]);
+                sc->dt_next = C.dtfix > 0.0 ? C.dtfix : C.sff * m;
+                sc->lim_prev = m;
+                sc->um_prev = o[0];
+            }
         }
     }
 }

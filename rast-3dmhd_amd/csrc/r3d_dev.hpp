@@ -36,7 +36,20 @@ struct DevScal {
     double dt, timc, timt, timi, umach;
     double dt_next, umach_next;
     int nit, pad;
+    // the last reduction's extrema: the least time-step limit over all cells
+    // and limits, and the largest ogamma*UU/T; they set the next screen (below)
+    double lim_prev, um_prev;
+    int dt_redo;  // the screened reduction could not prove its extrema: redo exactly
+    int pad2;
 };
+
+// Time-step screen of stage 3.  A cell's limits can only set the next dt if
+// one of them is at most the previous step's least limit times (1 + margin),
+// and its ogamma*UU/T can only set umach if it is at least the previous
+// largest times (1 - margin).  Cells that pass neither bound (tested with
+// products only, no division or square root) skip the exact evaluation; the
+// reduction then proves that its extrema are the exact ones, or has the step
+// redo them over every cell.  The margin is Consts::scr_up - 1.
 
 // Constants of the step: the reference's Derived/Params values that the
 // kernels read, combined exactly as the reference combines them
@@ -67,6 +80,7 @@ struct Consts {
     // BCON takes TU, DZTU, DZTB (:2852-2952)
     int lrem;
     double tu, dztu, dztb;
+    double scr_up, scr_dn;  // time-step screen: 1 + margin, 1 - margin
     int lrot, lmag, top_mode, half_t, izcon, bz_zero;
     int mag_bot;  // LMAG one-sided z stencils on the bottom wall (npez > 1,
                   // golden/fortran/3dmhdsub.f:1126, 1392, 1513 sit in an ELSE IF)
@@ -108,6 +122,7 @@ struct StageArgs {
     int stage;         // 0, 1, 2
     int zmode;         // 0: Z' = mix (X' + zeta dt F); 1: Z' = X (save_state); 2: no Z write
     int want_dt;       // stage 3 of a full step: emit dt partials of X'
+    int screen;        // stage 3: screen cells against the previous extrema (DevScal)
     int ntx, nty, nkc, kc, nblocks;
 };
 

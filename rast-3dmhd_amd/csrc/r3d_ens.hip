@@ -47,6 +47,7 @@ __device__ __forceinline__ StageArgs member_args(const EnsMember& e, int cur, in
     a.stage = stage;
     a.zmode = zmode;
     a.want_dt = want_dt;
+    a.screen = 0;
     a.ntx = e.ntx;
     a.nty = e.nty;
     a.nkc = e.nkc;

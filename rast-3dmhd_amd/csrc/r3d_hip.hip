@@ -97,8 +97,25 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt(const Geo g, const State s, c
 }
 
 __global__ __launch_bounds__(1024) void k_finish(DevScal* sc, const double* partial, int npart,
-                                                 const Consts C, int reduce, int advance) {
-    finish_block(sc, partial, npart, C, reduce, advance);
+                                                 const Consts C, int reduce, int advance,
+                                                 int screened) {
+    finish_block(sc, partial, npart, C, reduce, advance, screened);
+}
+
+// The exact dt pass after a screened reduction that could not prove its
+// extrema (DevScal::dt_redo); every block returns at once otherwise.
+__global__ __launch_bounds__(DT_THREADS) void k_dt_redo(const Geo g, const State s,
+                                                        const Consts C, const Metrics M,
+                                                        double* partial, const DevScal* sc) {
+    if (!sc->dt_redo) return;
+    dt_block(g, s, C, M, partial, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(1024) void k_finish_redo(DevScal* sc, const double* partial,
+                                                      int npart, const Consts C) {
+    // all threads read the flag before the reduction's first barrier; thread
+    // 0 clears it after the last
+    if (!sc->dt_redo) return;
+    finish_block(sc, partial, npart, C, 1, 0, 0);
 }
 
 // grid = (perimeter blocks, npz): planes 1..npz
@@ -402,6 +419,9 @@ void fill_consts(r3d_hip_ctx* c) {
     C.tu = f.tu;
     C.dztu = f.dztu;
     C.dztb = f.dztb;
+    const double margin = f.dt_screen != 0.0 ? f.dt_screen : 0.05;
+    C.scr_up = 1.0 + margin;
+    C.scr_dn = 1.0 - margin;
     C.gam[0] = f.gam1;
     C.gam[1] = f.gam2;
     C.gam[2] = f.gam3;
@@ -474,6 +494,7 @@ StageArgs stage_args(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
     a.stage = stage;
     a.zmode = zmode;
     a.want_dt = want_dt;
+    a.screen = stage == 2 && want_dt;
     a.ntx = c->ntx;
     a.nty = c->nty;
     a.nkc = c->nkc;
@@ -628,7 +649,7 @@ int launch_dt(r3d_hip_ctx* c) {
     if (rc) return rc;
     return timed(c, 3, [&] {
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, c->stream, c->scal, c->partial,
-                           DT_BLOCKS, c->C, 1, 0);
+                           DT_BLOCKS, c->C, 1, 0, 0);
     });
 }
 
@@ -698,9 +719,26 @@ int r3d::step_one(r3d_hip_ctx* c, int nstages, double* const* keep, double* cons
     }
     rc = timed(c, 3, [&] {
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, c->stream, c->scal, c->partial,
-                           c->nblocks + 2 * c->nbb, c->C, nstages == 3 ? 1 : 0, 1);
+                           c->nblocks + 2 * c->nbb, c->C, nstages == 3 ? 1 : 0, 1,
+                           nstages == 3 ? 1 : 0);
     });
     if (rc) return rc;
+    if (nstages == 3) {
+        // the screened reduction's fallback: an exact pass over the new state
+        // when its extrema were not proven (both kernels exit at once if so)
+        State s;
+        for (int f = 0; f < NXF; ++f) s.q[f] = c->xs[c->cur][f];
+        rc = timed(c, 2, [&] {
+            hipLaunchKernelGGL(k_dt_redo, dim3(DT_BLOCKS), dim3(DT_THREADS), 0, c->stream, c->g,
+                               s, c->C, c->M, c->partial, (const DevScal*)c->scal);
+        });
+        if (rc) return rc;
+        rc = timed(c, 3, [&] {
+            hipLaunchKernelGGL(k_finish_redo, dim3(1), dim3(1024), 0, c->stream, c->scal,
+                               c->partial, DT_BLOCKS, c->C);
+        });
+        if (rc) return rc;
+    }
     c->next_valid = nstages == 3;
     return R3D_OK;
 }
@@ -754,6 +792,8 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
             return set_err(R3D_EINVAL, "tile_rows %d: built are %s", f.tile_rows,
                            f.lmag ? "5 (LMAG)" : feature ? "8 (itcon 2, id, rlax)" : "8 and 9");
     }
+    if (!(f.dt_screen > -1.0))
+        return set_err(R3D_EINVAL, "dt_screen %g: the margin must be > -1", f.dt_screen);
     if (f.zchunk != 0 && f.zchunk != 8 && f.zchunk != 16 && f.zchunk != 32 && f.zchunk != 64 &&
         f.zchunk != 128)
         return set_err(R3D_EINVAL, "zchunk %d: one of 8, 16, 32, 64, 128", f.zchunk);

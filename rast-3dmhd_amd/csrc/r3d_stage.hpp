@@ -179,6 +179,57 @@ __device__ __forceinline__ void dt_terms(const Consts& C, double ru, double rv, 
     if constexpr (MAG) pm[4] = dmin(pm[4], 0.5 * dd * dd * C.rm);
 }
 
+// Constants of the time-step screen, formed once per block from the
+// previous extrema: the thresholds thr_um (ogamma*UU/T) and thr_lim (every
+// limit) with a relative margin of 1e-12 folded in, and the coefficients of
+// the limits (reference src/step.cpp:125-209).
+enum { SC_UM, SC_T, SC_T2, SC_TH, SC_VI, SC_RS, NSC };
+__device__ __forceinline__ void screen_consts(const Consts& C, double lim_prev, double um_prev,
+                                              double* k) {
+    const double up = 1.0 + 1e-12, dn = 1.0 - 1e-12;
+    const double thr_lim = lim_prev * C.scr_up, thr_um = um_prev * C.scr_dn;
+    k[SC_UM] = thr_um * dn;
+    k[SC_T] = thr_lim * up;
+    k[SC_T2] = thr_lim * thr_lim * up;
+    k[SC_TH] = 0.5 * C.repr * C.cv;
+    k[SC_VI] = 0.375 * C.re;
+    k[SC_RS] = 0.5 * C.rm;
+}
+
+// Whether a final state value may set an extremum of the next reduction:
+// one of its limits at most thr_lim, or ogamma*UU/T above thr_um (a cell
+// with ogamma*UU/T = thr_um = 0 cannot raise the maximum above the identity
+// 0).  With UU = m2/rho^2 the tests are multiplied out, so no division or
+// square root is needed: ogamma m2 > thr_um T rho^2, thermal dd^2 rho (0.5 repr cv) <=
+// thr_lim rk, viscous, resistive, and for dd/sqrt(fast) the bound fast =
+// (sqrt(UU) + sqrt(G))^2 <= 5 UU + 1.25 G (G = gamma T + b2, from
+// 2 sqrt(UU G) <= 4 UU + G/4).  Each side is a product of a few rounded
+// factors, so its relative error, like that of the exact limits, is far
+// below the margin 1e-12 in k: a cell that fails every test has every exact
+// limit above thr_lim and ogamma*UU/T at most thr_um.  The limit tests are
+// negated strict comparisons, m2 is NaN in the umach test only if a limit
+// test sees it too, and a non-positive rho, T or conductivity always hits,
+// so a NaN or an unphysical value takes the exact evaluation.
+template <bool MAG>
+__device__ __forceinline__ bool dt_screen(const Consts& C, double ru, double rv, double rw,
+                                          double ro, double tt, double bx, double by, double bz,
+                                          double dd, double rk, const double* k) {
+    const double m2 = ru * ru + rv * rv + rw * rw;
+    const double r2 = ro * ro;
+    const double d2 = dd * dd;
+    const double tr2 = tt * r2;
+    const double x = d2 * ro;
+    double g2 = C.gamma * tr2;  // G rho^2
+    if constexpr (MAG) g2 = g2 + (bx * bx + by * by + bz * bz) * C.obeta * ro;
+    bool hit = !(ro > 0.0 && tt > 0.0 && rk > 0.0);
+    hit |= C.og * m2 > k[SC_UM] * tr2;  // strict: a state at rest passes no cell
+    hit |= !(x * k[SC_TH] > k[SC_T] * rk);
+    hit |= !(x * k[SC_VI] > k[SC_T]);
+    hit |= !(d2 * r2 > k[SC_T2] * (5.0 * m2 + 1.25 * g2));
+    if constexpr (MAG) hit |= !(d2 * k[SC_RS] > k[SC_T]);
+    return hit;
+}
+
 // Write the block's dt extrema to partial[slot*NPART ...].  red must hold
 // NPART doubles per wave.
 template <int NW>
@@ -243,6 +294,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     constexpr int NPM = MAG ? NPART : NPART - 1;  // extrema in use (no resistive limit without B)
     __shared__ double pm_s[STAGE == 2 ? NPM : 1][STAGE == 2 ? TY : 1][64];
 
+    // stage 3 screen thresholds from the previous reduction (DevScal); read
+    // from LDS where they are used, so they hold no register across the loop
+    __shared__ double scr_s[NSC];
     // Stage 1 takes the dt reduced at the end of the previous step (or by the
     // standalone dt pass) and makes it the step's dt.
     const double dt = (STAGE == 0) ? a.scal->dt_next : a.scal->dt;
@@ -314,6 +368,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 zx[ZX_DEN + 5][q] = M.hrad[k0 + q];
             }
         }
+    }
+    if (STAGE == 2 && a.screen && threadIdx.x == 0) {
+        screen_consts(C, a.scal->lim_prev, a.scal->um_prev, scr_s);
     }
     if constexpr (HEAT) {
         for (int q = threadIdx.x; q < KCMAX; q += blockDim.x)
@@ -726,7 +783,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
 #undef LAPZ
 #undef SH
             if (STAGE == 2 && a.want_dt) {
-                // evaluated on every lane, folded in only on output cells
                 double q[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
                 asm volatile("" ::: "memory");  // keeps the LDS values in LDS
                 const double dd = dmin(ddxy_s[r][c], zm[2][kl]);
@@ -736,14 +792,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                     byv = xv[QBY];
                     bzv = xv[QBZ];
                 }
-                dt_terms<MAG>(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], bxv, byv, bzv, dd,
-                              zm[3][kl], q);
-                const bool use = outp && !bplane;
-                const double p0 = pm_s[0][r - 1][ln];
-                pm_s[0][r - 1][ln] = use ? dmax(p0, q[0]) : p0;
-                for (int v = 1; v < NPM; ++v) {
-                    const double pv = pm_s[v][r - 1][ln];
-                    pm_s[v][r - 1][ln] = use ? dmin(pv, q[v]) : pv;
+                const double rk = zm[3][kl];
+                // the exact limits on output cells that pass the screen (all
+                // with a.screen = 0); the others leave the extrema as they are
+                if (outp && !bplane &&
+                    (!a.screen || dt_screen<MAG>(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT],
+                                                 bxv, byv, bzv, dd, rk, scr_s))) {
+                    dt_terms<MAG>(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], bxv, byv, bzv,
+                                  dd, rk, q);
+                    pm_s[0][r - 1][ln] = dmax(pm_s[0][r - 1][ln], q[0]);
+                    for (int v = 1; v < NPM; ++v)
+                        pm_s[v][r - 1][ln] = dmin(pm_s[v][r - 1][ln], q[v]);
                 }
             }
         }

@@ -77,6 +77,18 @@ def test_create_rejects_unbuilt_tilings_without_a_device(bad):
     assert b"tile_rows" in L.r3d_hip_last_error() or b"zchunk" in L.r3d_hip_last_error()
 
 
+@pytest.mark.parametrize("margin", [-1.0, -2.0, float("nan")])
+def test_create_rejects_a_time_step_screen_margin_of_minus_one_or_below(margin):
+    """dt_screen scales the previous extrema by 1 +/- margin: it must be > -1."""
+    L = pkg.load()
+    cfg = pkg.Cfg()
+    cfg.npx, cfg.npy, cfg.npz, cfg.npey, cfg.npez = 16, 16, 32, 2, 2
+    cfg.dt_screen = margin
+    h = ctypes.c_void_p()
+    assert L.r3d_hip_create(ctypes.byref(cfg), ctypes.byref(h)) == -1 and not h.value
+    assert b"dt_screen" in L.r3d_hip_last_error()
+
+
 def test_ensemble_rejects_mixed_members_without_a_device():
     L = pkg.load()
     cfgs = (pkg.Cfg * 2)()

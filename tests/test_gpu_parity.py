@@ -180,3 +180,47 @@ def test_decomposition_invariance_of_device_state():
     for o in outs[1:]:
         for n in X_FIELDS:
             assert np.array_equal(o[n], outs[0][n])
+
+
+def noisy_start(args, seed=5, amp=0.5):
+    """The oracle's set-up state plus grid-scale velocity noise (amp * rho *
+    U(-0.5, 0.5) on each momentum component), which viscosity damps by ~1-2%
+    of max(ogamma*UU/T) per step; ghosts refreshed."""
+    orc = oracle.Sim(args)
+    rng = np.random.default_rng(seed)
+    for r in range(orc.nranks):
+        ro = orc.field(r, "ro")[1:-1, 1:-1, 1:-1]
+        for n in ("ru", "rv", "rw"):
+            f = orc.field(r, n)[1:-1, 1:-1, 1:-1]
+            f += ro * amp * (rng.random(f.shape) - 0.5)
+    orc.communicate()
+    return orc
+
+
+@pytest.mark.parametrize("margin", [0.0, 0.01, -0.5, 1e9],
+                         ids=["default", "tight", "exact_pass_every_step", "every_cell"])
+def test_dt_screen_matches_oracle(margin):
+    """The last stage's time-step screen, whatever its margin, gives the
+    oracle's dt and umach of every step bit for bit: the default margin, a
+    tight one (both outcomes occur: the oracle's umach^2 falls by more than
+    1% on some steps and by less on others), thresholds inside the previous
+    extrema (every step takes the exact pass) and one that evaluates every
+    cell."""
+    args = ["--npx", "32", "--npy", "24", "--npz", "40", "--npey", "2", "--npez", "2"]
+    say(f"[gpu] dt screen margin={margin}")
+    orc = noisy_start(args)
+    run = pkg.Run(args)
+    sim = run.make_sim(dt_screen=margin)
+    for r in range(orc.nranks):
+        sim.upload_rank(r, {n: orc.field(r, n) for n in X_FIELDS})
+    sc = orc.scalars()
+    sim.set_scalars(sc["dt"], sc["timc"], sc["timt"], sc["timi"], sc["umach"], sc["nit"])
+    um2 = []
+    for s in range(8):
+        sim.step(1)
+        orc.step(1)
+        compare_with_oracle(sim, orc, f"margin {margin}, step {s + 1}")
+        um2.append(orc.scalars()["umach"] ** 2)
+    ratios = [b / a for a, b in zip(um2[1:], um2[2:])]
+    assert min(ratios) < 0.99 < max(ratios), ratios
+    sim.close()
