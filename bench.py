@@ -15,6 +15,12 @@ roofline: the fused stage kernel, timed with HIP events on the library's
 cpu_baseline: the reference CPU build (oracle/_ref/3dmhd_nodump, MPI +
          OpenMP) on a bounded sample of the same workload, run before the GPU
          is touched; falls back to the C oracle when MPI or the build is absent.
+secondary: after the headline workload, the default run also measures the
+         metric's other workloads and adds them to its line: rotating MHD
+         252x252x512 (20 steps), the MHD adjoint 128x128x256 (backward :
+         forward wall-time ratio over 20 steps) and the 64-member ensemble
+         against the same members stepped one after another
+         (--no-secondary skips them).
 """
 import argparse
 import json
@@ -127,24 +133,35 @@ PHYSICS_NAMES = {"heat": "ITCON=2 embedded heat loss (TP 1.5, ZP 20, HH 4)",
                  "relax": "flux relaxation RLAX 0.5, TOFF 0.5 (ITCON=0, IZCON=1, uniform grid)"}
 
 
+def measure_adjoint(pkg, sizes, mhd, steps):
+    """Forward and backward wall times of the discrete adjoint (J = final
+    kinetic energy, checkpoints every 20 steps) after one untimed pair."""
+    npx, npy, npz, npey, npez = sizes
+    args = ["--npx", str(npx), "--npy", str(npy), "--npz", str(npz),
+            "--npey", str(npey), "--npez", str(npez)] + (MHD_FLAGS if mhd else [])
+    run = pkg.Run(args)
+    adj = pkg.Adjoint(run.cfg(), run.metrics(), 20)
+    adj.set_initial(run.state())
+    run.close()
+    log(f"[bench] adjoint: {steps} steps of {args}")
+    # warm-up: kernel loading and the first touch of every checkpoint and
+    # segment buffer
+    adj.forward(steps)
+    adj.backward()
+    J = adj.forward(steps)
+    adj.backward()
+    fwd_ms, bwd_ms = adj.timing()
+    adj.close()
+    log(f"[bench] J = {J:.9e}; forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms")
+    return fwd_ms, bwd_ms
+
+
 def run_adjoint(a):
     """Adjoint:forward time ratio of the hand-written discrete adjoint."""
     pkg = load_package()
-    args = ["--npx", str(a.npx), "--npy", str(a.npy), "--npz", str(a.npz),
-            "--npey", str(a.npey), "--npez", str(a.npez)] + (MHD_FLAGS if a.mhd else [])
-    run = pkg.Run(args)
     cells = a.npx * a.npy * a.npz
-    adj = pkg.Adjoint(run.cfg(), run.metrics(), 20)
-    adj.set_initial(run.state())
-    log(f"[bench] adjoint: {a.steps} steps of {args}")
-    # warm-up: kernel loading and the first touch of every checkpoint and
-    # segment buffer
-    adj.forward(a.steps)
-    adj.backward()
-    J = adj.forward(a.steps)
-    adj.backward()
-    fwd_ms, bwd_ms = adj.timing()
-    log(f"[bench] J = {J:.9e}; forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms")
+    fwd_ms, bwd_ms = measure_adjoint(pkg, (a.npx, a.npy, a.npz, a.npey, a.npez), a.mhd,
+                                     a.steps)
     out = {
         "metric": METRIC, "value": bwd_ms / fwd_ms, "unit": "adjoint:forward time ratio",
         "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": bwd_ms / a.steps,
@@ -226,11 +243,10 @@ def run_checkpoint(a):
     print(json.dumps(out), flush=True)
 
 
-def run_ensemble(a):
-    """Aggregate cell-updates/s of the batched ensemble (autotuned tiles, graph
-    replay) next to the same members stepped one after another."""
-    pkg = load_package()
-    members = pkg.ensemble_members(a.ensemble)
+def measure_ensemble(pkg, nmem, steps, warmup):
+    """The batched ensemble (autotuned tiles, graph replay) and the same
+    members stepped one after another through the plain path."""
+    members = pkg.ensemble_members(nmem)
     runs = [pkg.Run(m) for m in members]
     cells = sum(r.npx * r.npy * r.npz for r in runs)
     log(f"[bench] ensemble of {len(runs)} members, {cells} cells")
@@ -238,26 +254,37 @@ def run_ensemble(a):
     ms_tuned = ens.autotune(3)
     ty, kc, nt = ens.tiles()
     log(f"[bench] autotuned tiles: ty={ty} kc={kc} ({nt} tiles/stage), {ms_tuned:.3f} ms/step")
-    ens.step(a.warmup)
+    ens.step(warmup)
     ens.sync()
     t0 = time.perf_counter()
-    ens.step(a.steps)
+    ens.step(steps)
     ens.sync()
     wall = time.perf_counter() - t0
     mem = ens.memory_bytes()
     ens.close()
-    value = cells * 3 * a.steps / wall
-    log(f"[bench] ensemble {wall / a.steps * 1e3:.3f} ms/step; sequential reference next")
+    log(f"[bench] ensemble {wall / steps * 1e3:.3f} ms/step; sequential reference next")
     seq = 0.0
     for r in runs:
         sim = r.make_sim()
-        sim.step(a.warmup)
+        sim.step(warmup)
         sim.sync()
         t0 = time.perf_counter()
-        sim.step(a.steps)
+        sim.step(steps)
         sim.sync()
         seq += time.perf_counter() - t0
         sim.close()
+        r.close()
+    return {"members": len(runs), "cells": cells, "wall": wall, "seq": seq, "tiles": (ty, kc),
+            "mem": mem}
+
+
+def run_ensemble(a):
+    """Aggregate cell-updates/s of the batched ensemble (autotuned tiles, graph
+    replay) next to the same members stepped one after another."""
+    pkg = load_package()
+    e = measure_ensemble(pkg, a.ensemble, a.steps, a.warmup)
+    cells, wall, seq, (ty, kc), mem = e["cells"], e["wall"], e["seq"], e["tiles"], e["mem"]
+    value = cells * 3 * a.steps / wall
     seq_value = cells * 3 * a.steps / seq
     achieved = BYTES_PER_CELL_STAGE * cells * 3 * a.steps / wall / 1e9
     out = {
@@ -265,7 +292,7 @@ def run_ensemble(a):
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded per-member polytropes and ran2 perturbations",
-        "config": {"workload": f"ensemble of {len(runs)} hydro members (48..96)^2 x 96..192, "
+        "config": {"workload": f"ensemble of {e['members']} hydro members (48..96)^2 x 96..192, "
                                "Re in [1,4], theta in [0.1,0.3], virtual 1x2 each",
                    "global_cells": cells, "tiles": [ty, kc], "graph": True,
                    "device_bytes": mem, "sequential_value": seq_value,
@@ -310,6 +337,9 @@ def main():
                     help="plain (default); heat: the ITCON=2 embedded heat loss; density: "
                          "density diffusion ID=1 (uniform grid); relax: flux relaxation")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the MHD, adjoint and ensemble figures that the default run "
+                         "adds to its line after the headline workload")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="before the run, time one device-to-device copy of this many GiB "
                          "(a plain copy of known size, used to scale profiler byte counters)")
@@ -355,33 +385,9 @@ def main():
     pkg = load_package()
     args = ["--npx", str(a.npx), "--npy", str(a.npy), "--npz", str(a.npz),
             "--npey", str(a.npey), "--npez", str(a.npez)] + extra
-    log(f"[bench] host setup {' '.join(args)}")
-    t0 = time.perf_counter()
-    run = pkg.Run(args)
-    log(f"[bench] host setup done in {time.perf_counter() - t0:.1f}s; uploading")
-    sim = run.make_sim(a.variant)
-    run.close()
-    mem = sim.memory_bytes()
-    log(f"[bench] device state {mem / 2**30:.1f} GiB; warm-up {a.warmup} steps")
-    sim.step(a.warmup)
-    sim.sync()
-    sim.set_timing(True)
-    log(f"[bench] timing {a.steps} steps")
-    t0 = time.perf_counter()
-    sim.step(a.steps)
-    sim.sync()
-    wall = time.perf_counter() - t0
-    n_stage, ms_stage = sim.kernel_stats(0)
-    n_bound, ms_bound = sim.kernel_stats(1)
-    sim.set_timing(False)
-    sc = sim.scalars()
-    log(f"[bench] nit={sc['nit']} dt={sc['dt']:.6g} umach={sc['umach']:.6g}; "
-        f"stage kernel {n_stage} launches avg {ms_stage / max(n_stage, 1):.3f} ms, "
-        f"boundary kernel avg {ms_bound / max(n_bound, 1):.3f} ms")
-
+    wall, avg_stage_s, mem = measure_steps(pkg, args, a.variant, a.steps, a.warmup)
     cells = a.npx * a.npy * a.npz
     value = cells * 3 * a.steps / wall
-    avg_stage_s = ms_stage / max(n_stage, 1) / 1e3
     achieved = bpc * cells / avg_stage_s / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")
@@ -421,7 +427,70 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
     }
+    if plain and not a.mhd and a.variant == 0 and not a.no_secondary and (
+            a.npx, a.npy, a.npz, a.npey, a.npez) == (504, 504, 2048, 4, 2):
+        out["secondary"] = secondary(pkg)
     print(json.dumps(out), flush=True)
+
+
+def measure_steps(pkg, args, variant, steps, warmup):
+    """Wall seconds of `steps` timed steps after `warmup`, the stage kernel's
+    average launch seconds (HIP events on the library's stream) and the
+    device bytes of the run."""
+    log(f"[bench] host setup {' '.join(args)}")
+    t0 = time.perf_counter()
+    run = pkg.Run(args)
+    log(f"[bench] host setup done in {time.perf_counter() - t0:.1f}s; uploading")
+    sim = run.make_sim(variant)
+    run.close()
+    mem = sim.memory_bytes()
+    log(f"[bench] device state {mem / 2**30:.1f} GiB; warm-up {warmup} steps")
+    sim.step(warmup)
+    sim.sync()
+    sim.set_timing(True)
+    log(f"[bench] timing {steps} steps")
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    n_stage, ms_stage = sim.kernel_stats(0)
+    n_bound, ms_bound = sim.kernel_stats(1)
+    sim.set_timing(False)
+    sc = sim.scalars()
+    log(f"[bench] nit={sc['nit']} dt={sc['dt']:.6g} umach={sc['umach']:.6g}; "
+        f"stage kernel {n_stage} launches avg {ms_stage / max(n_stage, 1):.3f} ms, "
+        f"boundary kernel avg {ms_bound / max(n_bound, 1):.3f} ms")
+    sim.close()
+    return wall, ms_stage / max(n_stage, 1) / 1e3, mem
+
+
+def secondary(pkg):
+    """The other workloads of the metric, measured after the headline one in
+    the same run: rotating MHD 252x252x512 (2x4), the MHD adjoint
+    128x128x256 (2x4, 20 steps) and the 64-member ensemble."""
+    out = {}
+    cells = 252 * 252 * 512
+    args = ["--npx", "252", "--npy", "252", "--npz", "512", "--npey", "2", "--npez", "4"]
+    wall, avg_stage_s, _ = measure_steps(pkg, args + MHD_FLAGS, 0, 20, 5)
+    ach = BYTES_PER_CELL_STAGE_MHD * cells / avg_stage_s / 1e9
+    out["mhd"] = {"workload": "rotating MHD 252x252x512, virtual 2x4, bit-exact", "steps": 20,
+                  "value": cells * 3 * 20 / wall, "unit": "cell-updates/s",
+                  "ms_per_step": wall / 20 * 1e3, "stage_kernel_achieved_gbs": ach,
+                  "stage_kernel_frac": ach / HBM_PEAK_GBS}
+    fwd_ms, bwd_ms = measure_adjoint(pkg, (128, 128, 256, 2, 4), True, 20)
+    out["adjoint"] = {"workload": "MHD adjoint 128x128x256, virtual 2x4, 20 steps, checkpoints "
+                                  "every 20, J = final kinetic energy",
+                      "value": bwd_ms / fwd_ms, "unit": "adjoint:forward time ratio",
+                      "forward_ms": fwd_ms, "backward_ms": bwd_ms}
+    e = measure_ensemble(pkg, 64, 20, 5)
+    ev = e["cells"] * 3 * 20 / e["wall"]
+    out["ensemble"] = {"workload": "ensemble of 64 hydro members (48..96)^2 x 96..192, Re in "
+                                   "[1,4], theta in [0.1,0.3], graph replay", "steps": 20,
+                       "value": ev, "unit": "cell-updates/s",
+                       "sequential_value": e["cells"] * 3 * 20 / e["seq"],
+                       "speedup_vs_sequential": e["seq"] / e["wall"],
+                       "frac": BYTES_PER_CELL_STAGE * ev / (HBM_PEAK_GBS * 1e9)}
+    return out
 
 
 if __name__ == "__main__":
