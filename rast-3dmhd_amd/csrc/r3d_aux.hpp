@@ -213,9 +213,8 @@ __device__ __forceinline__ void finish_block(DevScal* sc, const double* partial,
             }
             double m = dmin(dmin(o[1], o[2]), o[3]);
             if (C.lmag) m = dmin(m, o[4]);
-            // the thresholds exactly as the stage kernel formed them
-            const bool proven = !screened || (m <= sc->lim_prev * C.scr_up &&
-                                              o[0] >= sc->um_prev * C.scr_dn);
+            const bool proven =
+                !screened || screen_proven(C, m, o[0], sc->lim_prev, sc->um_prev);
             sc->dt_redo = proven ? 0 : 1;
             if (proven) {
                 sc->umach_next = sqrt(o[0]);
