@@ -11,7 +11,9 @@ value  = interior cells * 3 stages * K / wall seconds of the timed region
 roofline: the fused stage kernel, timed with HIP events on the library's
          stream over the timed region; algorithmic bytes per launch =
          133.33 B per cell (the 400 B/cell/step of reading X, writing X' and
-         Z, reading Z, ... over 5 FP64 fields, split over 3 stages).
+         Z, reading Z, ... over 5 FP64 fields, split over 3 stages); also
+         against the measured bandwidth of a 4 GiB device-to-device copy
+         (measured_copy_gbs, frac_of_copy).
 cpu_baseline: the reference CPU build (oracle/_ref/3dmhd_nodump, MPI +
          OpenMP) on a bounded sample of the same workload, run before the GPU
          is touched; falls back to the C oracle when MPI or the build is absent.
@@ -107,6 +109,40 @@ def cpu_baseline(npx, npy, npey, npez, cores):
     dt = time.perf_counter() - t0
     return {"value": npx * npy * nz * 3 * 2 / dt, "unit": "cell-updates/s", "cores": cores,
             "kind": "port", "sample": f"C oracle (OpenMP), {npx}x{npy}x{nz}, 2 steps"}
+
+
+def measure_copy_gbs(nbytes=4 << 30, reps=5):
+    """Device-to-device copy bandwidth (bytes read + written per second) of
+    the runtime's own copy kernel, timed with HIP events: the practical
+    streaming ceiling the stage kernel's rate is also compared with."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) != 0:
+        return None
+    try:
+        if hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) != 0:
+            return None
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        d2d = 3  # hipMemcpyDeviceToDevice
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), d2d)
+        best = None
+        for _ in range(reps):
+            hip.hipEventRecord(e0, None)
+            hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), d2d, None)
+            hip.hipEventRecord(e1, None)
+            hip.hipEventSynchronize(e1)
+            ms = ctypes.c_float()
+            hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+            best = ms.value if best is None else min(best, ms.value)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        hip.hipFree(b)
+        return 2.0 * nbytes / (best / 1e3) / 1e9
+    finally:
+        hip.hipFree(a)
 
 
 def load_package():
@@ -427,6 +463,10 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
     }
+    copy = measure_copy_gbs()
+    if copy:
+        out["roofline"]["measured_copy_gbs"] = copy
+        out["roofline"]["frac_of_copy"] = achieved / copy
     if plain and not a.mhd and a.variant == 0 and not a.no_secondary and (
             a.npx, a.npy, a.npz, a.npey, a.npez) == (504, 504, 2048, 4, 2):
         out["secondary"] = secondary(pkg)
