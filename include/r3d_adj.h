@@ -62,6 +62,14 @@ int r3d_adj_set_fixed_dt(r3d_adj_ctx* a, double dt);
 int r3d_adj_timing(r3d_adj_ctx* a, double* forward_ms, double* backward_ms);
 /* Final state of the last forward (as r3d_hip_download_global). */
 int r3d_adj_download_final(r3d_adj_ctx* a, double* const* fields);
+/* r3d_adj_set_initial and r3d_adj_backward with the fields and gradients in
+ * device memory of the context's GPU (same dense global layout), for callers
+ * that hold the state on the GPU already, e.g. a torch binding: no host copy
+ * is made.  The caller's writes to the inputs must be complete (its stream
+ * synchronized) before the call; the outputs are complete when it returns. */
+int r3d_adj_set_initial_device(r3d_adj_ctx* a, const double* const* fields);
+int r3d_adj_backward_device(r3d_adj_ctx* a, double* const* grad, double* dJ_dre,
+                            double* dJ_dtheta);
 
 #ifdef __cplusplus
 }

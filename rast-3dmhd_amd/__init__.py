@@ -72,6 +72,14 @@ def load():
         return _hip
     if not os.path.exists(HIP_LIB):
         raise RuntimeError(f"HIP library not built: {HIP_LIB} (run make -C rast-3dmhd_amd)")
+    # torch, when installed, is loaded first: it ships a HIP runtime under the
+    # soname this library links (libamdhip64.so.7), the first one loaded
+    # serves the whole process, and torch's GPU tensors (the adjoint's torch
+    # binding) need torch's own
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(HIP_LIB)
     vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
     L.r3d_hip_last_error.restype = ctypes.c_char_p
@@ -123,6 +131,9 @@ def load():
         "r3d_adj_set_fixed_dt": [vp, ctypes.c_double],
         "r3d_adj_dJ_dtb": [vp, ctypes.POINTER(ctypes.c_double)],
         "r3d_adj_download_final": [vp, ctypes.POINTER(dp)],
+        "r3d_adj_set_initial_device": [vp, ctypes.POINTER(vp)],
+        "r3d_adj_backward_device": [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -440,6 +451,22 @@ class Adjoint:
             assert a.shape == self.shape
         ptrs = (ctypes.POINTER(ctypes.c_double) * len(arrs))(*[_dptr(a) for a in arrs])
         _check(load().r3d_adj_set_initial(self.h, ptrs))
+
+    def set_initial_device(self, ptrs):
+        """X0 from device memory: ptrs[name] = address of a dense global array
+        (float64, shape self.shape) on the context's GPU, for every name in
+        self.names; the caller's writes must be complete."""
+        arr = (ctypes.c_void_p * len(self.names))(*[ptrs[n] for n in self.names])
+        _check(load().r3d_adj_set_initial_device(self.h, arr))
+
+    def backward_device(self, ptrs):
+        """backward() into device memory: ptrs[name] = address of a dense
+        global array (float64, shape self.shape) on the context's GPU for each
+        gradient wanted (others are skipped).  Returns (dJ/dRe, dJ/dtheta)."""
+        arr = (ctypes.c_void_p * len(self.names))(*[ptrs.get(n) for n in self.names])
+        dre, dth = ctypes.c_double(), ctypes.c_double()
+        _check(load().r3d_adj_backward_device(self.h, arr, ctypes.byref(dre), ctypes.byref(dth)))
+        return dre.value, dth.value
 
     def forward(self, nsteps):
         J = ctypes.c_double()

@@ -20,7 +20,11 @@ differentiable maps to J = the final kinetic energy:
   profile as well as the boundary conditions.
 
 The forward is the plain bit-exact time step on the GPU; the backward is the
-hand-written adjoint.  The static profile's theta derivative is the central
+hand-written adjoint.  With GPU tensors the initial state and the gradients
+stay in device memory (r3d_adj_set_initial_device / r3d_adj_backward_device:
+the setup's state is copied to the device once per problem, and each call
+only writes the tensors into a device copy of it); CPU tensors go through
+host arrays.  The static profile's theta derivative is the central
 difference of the host setup (a smooth ODE solution; step 1e-6 theta).
 theta also sets the conductivity profile when the conductivity jump is on
 (--pzp != 0); that dependence is not differentiated (the metrics are held
@@ -44,6 +48,18 @@ class KineticEnergyRun:
         self.every = checkpoint_every
         self._key = None
         self._adj = None
+        self._base_dev = None  # the setup's state as device tensors, made on first use
+
+    def base_on(self, device):
+        """A fresh copy of the setup's full initial state (ghosts included) as
+        float64 tensors on `device` (the device copy of the host state is made
+        once and cloned per call)."""
+        if device.type == "cpu":
+            return {k: torch.from_numpy(v.copy()) for k, v in self.base.items()}
+        if self._base_dev is None or self._base_dev["tt"].device != device:
+            self._base_dev = {k: torch.from_numpy(np.ascontiguousarray(v)).to(device)
+                              for k, v in self.base.items()}
+        return {k: v.clone() for k, v in self._base_dev.items()}
 
     def close(self):
         """Free the cached adjoint context (device checkpoints and segment
@@ -51,6 +67,7 @@ class KineticEnergyRun:
         if self._adj is not None:
             self._adj.close()
             self._adj, self._key = None, None
+        self._base_dev = None
         self.run.close()
 
     def adjoint(self, re, theta):
@@ -102,16 +119,49 @@ class KineticEnergyRun:
         return _KineticEnergyState.apply(fields["ro"], fields["tt"], *b, re, theta, tb, self)
 
 
+def _set_initial(adj, fields):
+    """X0 from full-state tensors: device pointers for GPU tensors (the
+    writes of torch's stream completed first), host arrays for CPU ones."""
+    if fields["tt"].is_cuda:
+        torch.cuda.synchronize(fields["tt"].device)
+        adj.set_initial_device({n: fields[n].data_ptr() for n in adj.names})
+    else:
+        adj.set_initial({n: fields[n].numpy() for n in adj.names})
+
+
+def _devices(*xs):
+    """The device of each tensor argument (None for plain numbers), so that
+    every gradient is returned where its input lives."""
+    return [x.device if torch.is_tensor(x) else None for x in xs]
+
+
+def _scalar(v, device):
+    return torch.tensor(v, dtype=torch.float64, device=device)
+
+
+def _backward(adj, names, device):
+    """Gradients of the wanted primaries as full-state tensors on `device`,
+    and dJ/dRe, dJ/dtheta."""
+    if device.type != "cpu":
+        out = {n: torch.zeros(adj.shape, dtype=torch.float64, device=device) for n in names}
+        torch.cuda.synchronize(device)
+        dre, dth = adj.backward_device({n: out[n].data_ptr() for n in names})
+        return out, dre, dth
+    grads, dre, dth = adj.backward()
+    return {n: torch.from_numpy(grads[n]) for n in names}, dre, dth
+
+
 class _KineticEnergy(torch.autograd.Function):
     @staticmethod
     def forward(ctx, tt0, re, theta, prob):
         adj = prob.adjoint(float(re), float(theta))
-        fields = {k: v.copy() for k, v in prob.base.items()}
-        fields["tt"][1:-1, 1:-1, 1:-1] = tt0.detach().to("cpu", torch.float64).numpy()
-        adj.set_initial(fields)
+        fields = prob.base_on(tt0.device)
+        fields["tt"][1:-1, 1:-1, 1:-1] = tt0.detach().to(torch.float64)
+        _set_initial(adj, fields)
         J = adj.forward(prob.nsteps)
         ctx.adj = adj
         ctx.dev = tt0.device
+        ctx.pdev = _devices(re, theta)
         return torch.tensor(J, dtype=torch.float64, device=tt0.device)
 
     @staticmethod
@@ -119,10 +169,11 @@ class _KineticEnergy(torch.autograd.Function):
         # the context belongs to prob (prob.adjoint caches one, reused while
         # Re and theta stay the same; prob.close() frees it)
         adj, ctx.adj = ctx.adj, None
-        grads, dre, dth = adj.backward()
-        g = torch.from_numpy(grads["tt"][1:-1, 1:-1, 1:-1].copy()).to(ctx.dev)
-        return (gJ * g, gJ * torch.tensor(dre, dtype=torch.float64, device=ctx.dev),
-                gJ * torch.tensor(dth, dtype=torch.float64, device=ctx.dev), None)
+        grads, dre, dth = _backward(adj, ["tt"], ctx.dev)
+        g = grads["tt"][1:-1, 1:-1, 1:-1]
+        dv_re, dv_th = ctx.pdev
+        return (gJ * g, gJ.to(dv_re or ctx.dev) * _scalar(dre, dv_re or ctx.dev),
+                gJ.to(dv_th or ctx.dev) * _scalar(dth, dv_th or ctx.dev), None)
 
 
 class _StaticState(torch.autograd.Function):
@@ -143,7 +194,8 @@ class _StaticState(torch.autograd.Function):
         total = 0.0
         for gi, p_, m_ in zip(g[:5], ap, am):
             if gi is not None:
-                total += float(torch.sum(gi.cpu() * torch.from_numpy((p_ - m_) / (2 * h))))
+                dq = torch.from_numpy((p_ - m_) / (2 * h)).to(gi.device)
+                total += float(torch.sum(gi * dq))
         if g[5] is not None:
             total += float(g[5]) * (tbp - tbm) / (2 * h)
         return torch.tensor(total, dtype=torch.float64, device=ctx.dev), None
@@ -155,21 +207,22 @@ class _KineticEnergyState(torch.autograd.Function):
         cfg = cfg_with(prob.cfg, re=float(re), theta=float(theta), tb=float(tb))
         adj = Adjoint(cfg, prob.metrics, prob.every)
         adj.set_fixed_dt(prob.fixed_dt)
-        fields = {k: v.copy() for k, v in prob.base.items()}
+        fields = prob.base_on(tt0.device)
         sl = (slice(1, -1),) * 3
         ins = {"ro": ro0, "tt": tt0, "bx": bx0, "by": by0, "bz": bz0}
         for n in adj.names:
             if n in ins:
-                fields[n][sl] = ins[n].detach().to("cpu", torch.float64).numpy()
-        adj.set_initial(fields)
+                fields[n][sl] = ins[n].detach().to(tt0.device, torch.float64)
+        _set_initial(adj, fields)
         J = adj.forward(prob.nsteps)
         ctx.adj, ctx.dev = adj, tt0.device
+        ctx.pdev = _devices(re, theta, tb)
         return torch.tensor(J, dtype=torch.float64, device=tt0.device)
 
     @staticmethod
     def backward(ctx, gJ):
         adj, ctx.adj = ctx.adj, None
-        grads, dre, dth = adj.backward()
+        grads, dre, dth = _backward(adj, adj.names, ctx.dev)
         dtb = adj.dJ_dtb()
         adj.close()
         sl = (slice(1, -1),) * 3
@@ -177,9 +230,12 @@ class _KineticEnergyState(torch.autograd.Function):
         def g(n):
             if n not in grads:
                 return None
-            return gJ * torch.from_numpy(grads[n][sl].copy()).to(ctx.dev)
+            return gJ * grads[n][sl]
 
-        def sc(v):
-            return gJ * torch.tensor(v, dtype=torch.float64, device=ctx.dev)
+        def sc(v, d):
+            d = d or ctx.dev
+            return gJ.to(d) * _scalar(v, d)
 
-        return (g("ro"), g("tt"), g("bx"), g("by"), g("bz"), sc(dre), sc(dth), sc(dtb), None)
+        dv = ctx.pdev
+        return (g("ro"), g("tt"), g("bx"), g("by"), g("bz"), sc(dre, dv[0]), sc(dth, dv[1]),
+                sc(dtb, dv[2]), None)

@@ -972,16 +972,28 @@ int r3d_adj_set_metrics(r3d_adj_ctx* a, const r3d_hip_metrics* m) {
     return r3d_hip_set_metrics(a->work, m);
 }
 
-int r3d_adj_set_initial(r3d_adj_ctx* a, const double* const* fields) {
+namespace {
+int set_initial(r3d_adj_ctx* a, const double* const* fields, bool device) {
     if (!a || !fields) return set_err(R3D_EINVAL, "null argument");
     for (int f = 0; f < a->nf; ++f) {
         if (!fields[f]) return set_err(R3D_EINVAL, "field %d missing", f);
-        int rc = global_copy(a->work, a->x0 + f * a->fe, const_cast<double*>(fields[f]), false);
+        double* src = const_cast<double*>(fields[f]);
+        int rc = device ? global_copy_device(a->work, a->x0 + f * a->fe, src, false)
+                        : global_copy(a->work, a->x0 + f * a->fe, src, false);
         if (rc) return rc;
     }
     a->nsteps = 0;
     a->seg_outer = false;
     return R3D_OK;
+}
+}  // namespace
+
+int r3d_adj_set_initial(r3d_adj_ctx* a, const double* const* fields) {
+    return set_initial(a, fields, false);
+}
+
+int r3d_adj_set_initial_device(r3d_adj_ctx* a, const double* const* fields) {
+    return set_initial(a, fields, true);
 }
 
 int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J) {
@@ -1013,7 +1025,8 @@ int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J) {
     return R3D_OK;
 }
 
-int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double* dJ_dth) {
+namespace {
+int backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double* dJ_dth, bool device) {
     if (!a) return set_err(R3D_EINVAL, "null adjoint context");
     if (a->nsteps < 1) return set_err(R3D_ESTATE, "no forward run to differentiate");
     r3d_hip_ctx* w = a->work;
@@ -1088,11 +1101,23 @@ int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double
     const double g_re = acc[0] * (-C.ore * C.ore) + acc[1] * (-C.tk / C.re) + acc[4];
     if (grad)
         for (int f = 0; f < a->nf; ++f)
-            if (grad[f] && (rc = global_copy(w, L + f * a->fe, grad[f], true))) return rc;
+            if (grad[f] && (rc = device ? global_copy_device(w, L + f * a->fe, grad[f], true)
+                                        : global_copy(w, L + f * a->fe, grad[f], true)))
+                return rc;
     if (dJ_dre) *dJ_dre = g_re;
     if (dJ_dth) *dJ_dth = acc[3];
     a->dJ_dtb = acc[5];
     return R3D_OK;
+}
+}  // namespace
+
+int r3d_adj_backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double* dJ_dth) {
+    return backward(a, grad, dJ_dre, dJ_dth, false);
+}
+
+int r3d_adj_backward_device(r3d_adj_ctx* a, double* const* grad, double* dJ_dre,
+                            double* dJ_dth) {
+    return backward(a, grad, dJ_dre, dJ_dth, true);
 }
 
 int r3d_adj_set_fixed_dt(r3d_adj_ctx* a, double dt) {

@@ -94,6 +94,9 @@ int dev_alloc(r3d_hip_ctx* c, double** p, size_t n);
 // host array of (npx+2)(npy+2)(npz+2) doubles, x fastest: packed on the device
 // and moved in one transfer.  Synchronises c->stream.
 int global_copy(r3d_hip_ctx* c, double* field, double* host, bool to_host);
+// The same between a padded field and a dense global array in device memory
+// (no staging); returns after the copy has completed.
+int global_copy_device(r3d_hip_ctx* c, double* field, double* dense, bool to_dense);
 // Write the periodic ghost images of c's current X (downloads read them).
 int fill_ghosts(r3d_hip_ctx* c);
 // All virtual ranks and the metrics uploaded.

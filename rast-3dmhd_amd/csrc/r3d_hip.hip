@@ -676,6 +676,17 @@ int r3d::global_copy(r3d_hip_ctx* c, double* field, double* host, bool to_host) 
     return R3D_OK;
 }
 
+int r3d::global_copy_device(r3d_hip_ctx* c, double* field, double* dense, bool to_dense) {
+    const r3d_hip_cfg& f = c->cfg;
+    const size_t n = (size_t)(f.npx + 2) * (f.npy + 2) * (f.npz + 2);
+    const dim3 grid((unsigned)std::min<size_t>(4096, (n + 255) / 256)), blk(256);
+    hipLaunchKernelGGL(k_global_copy, grid, blk, 0, c->stream, field, dense, c->g,
+                       to_dense ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return R3D_OK;
+}
+
 int r3d::fill_ghosts(r3d_hip_ctx* c) {
     State s;
     for (int f = 0; f < NXF; ++f) s.q[f] = c->xs[c->cur][f];

@@ -3,6 +3,15 @@ import sys
 
 import pytest
 
+# torch (when present) is imported before the library is first loaded: both
+# link a HIP runtime under the soname libamdhip64.so.7, the first one loaded
+# serves the process, and torch's GPU tensors need its own (the torch binding
+# of the adjoint is tested with them).
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, ROOT)

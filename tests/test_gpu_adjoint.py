@@ -189,6 +189,49 @@ def test_torch_function_gradient_of_the_perturbation():
     assert re.grad is not None and th.grad is not None
 
 
+def test_torch_function_on_gpu_tensors_equals_host_tensors():
+    """The torch binding with GPU tensors (state and gradients in device
+    memory, r3d_adj_set_initial_device / r3d_adj_backward_device) gives J
+    bit for bit as with CPU tensors (host arrays), and the gradients to the
+    run-to-run reproducibility of the transpose, whose atomic sums have no
+    fixed order (1e-12 of each field's largest entry)."""
+    torch = pytest.importorskip("torch")
+    at = importlib.import_module("rast3dmhd_amd.adjoint_torch")
+    say("[gpu] torch adjoint: GPU tensors against CPU tensors")
+    prob = at.KineticEnergyRun(ARGS, NSTEPS, EVERY)
+    out = []
+    for dev in (torch.device("cpu"), torch.device("cuda:0")):
+        tt0 = prob.initial_tt().to(dev).requires_grad_(True)
+        re = torch.tensor(prob.cfg.re, dtype=torch.float64, device=dev, requires_grad=True)
+        th = torch.tensor(prob.cfg.theta, dtype=torch.float64, device=dev, requires_grad=True)
+        J = prob(tt0, re, th)
+        J.backward()
+        assert J.device.type == dev.type and tt0.grad.device.type == dev.type
+        out.append((float(J.detach()), tt0.grad.cpu().numpy(), float(re.grad), float(th.grad)))
+    (ja, ga, ra, ta), (jb, gb, rb, tb) = out
+
+    def close(a, b):
+        return np.abs(np.asarray(a) - np.asarray(b)).max() <= 1e-12 * np.abs(a).max()
+
+    assert ja == jb
+    assert close(ga, gb) and close(ra, rb) and close(ta, tb)
+    # the state form, B and the density included
+    st = prob.static_state(torch.tensor(prob.cfg.theta, dtype=torch.float64))
+    res = []
+    for dev in (torch.device("cpu"), torch.device("cuda:0")):
+        f = {n: st[n].detach().to(dev).clone().requires_grad_(True)
+             for n in ("ro", "tt", "bx", "by", "bz")}
+        J = prob.kinetic_energy(f, torch.tensor(prob.cfg.re, dtype=torch.float64, device=dev),
+                                torch.tensor(prob.cfg.theta, dtype=torch.float64, device=dev),
+                                torch.tensor(prob.cfg.tb, dtype=torch.float64, device=dev))
+        J.backward()
+        res.append((float(J.detach()), f["ro"].grad.cpu().numpy(), f["tt"].grad.cpu().numpy()))
+    assert res[0][0] == res[1][0]
+    for a, b in zip(res[0][1:], res[1][1:]):
+        assert close(a, b)
+    prob.close()
+
+
 MAG_ARGS = ["--npx", "16", "--npy", "16", "--npz", "32", "--npey", "2", "--npez", "2",
             "--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2",
             "--bzp", "10", "--lrot", "1", "--ry", "0.1", "--ang", "0"] + SMOOTH

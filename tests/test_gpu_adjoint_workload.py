@@ -43,11 +43,13 @@ def workload():
     say("[gpu] adjoint workload: MHD 128x128x256, 200 steps")
     prob = at.KineticEnergyRun(ARGS, NSTEPS, EVERY)
     static = pkg.Run(ARGS + ["--ampt", "0"]).state()["tt"][1:-1, 1:-1, 1:-1]
-    ts = torch.from_numpy(static.copy())
-    delta0 = prob.initial_tt() / ts - 1.0
+    # GPU tensors: the state and the gradients stay in device memory
+    dev = torch.device("cuda:0")
+    ts = torch.from_numpy(static.copy()).to(dev)
+    delta0 = prob.initial_tt().to(dev) / ts - 1.0
     delta = delta0.clone().requires_grad_(True)
-    re = torch.tensor(prob.cfg.re, dtype=torch.float64, requires_grad=True)
-    th = torch.tensor(prob.cfg.theta, dtype=torch.float64, requires_grad=True)
+    re = torch.tensor(prob.cfg.re, dtype=torch.float64, device=dev, requires_grad=True)
+    th = torch.tensor(prob.cfg.theta, dtype=torch.float64, device=dev, requires_grad=True)
     J = prob(ts * (1.0 + delta), re, th)
     J.backward()
     say(f"[gpu]   J = {float(J.detach()):.10e}, dJ/dRe = {float(re.grad):.10e}, "
@@ -55,8 +57,8 @@ def workload():
 
     def J_at(dl, r, t):
         with torch.no_grad():
-            return float(prob(ts * (1.0 + dl), torch.tensor(r, dtype=torch.float64),
-                              torch.tensor(t, dtype=torch.float64)))
+            return float(prob(ts * (1.0 + dl), torch.tensor(r, dtype=torch.float64, device=dev),
+                              torch.tensor(t, dtype=torch.float64, device=dev)))
 
     return torch, prob, delta0, delta.grad.clone(), float(re.grad), float(th.grad), J_at
 
@@ -67,7 +69,7 @@ def test_perturbation_gradient_in_16_random_directions(workload):
     eps, worst = 1e-6, 0.0
     re, th = prob.cfg.re, prob.cfg.theta
     for n in range(16):
-        d = torch.from_numpy(rng.uniform(-1, 1, tuple(delta0.shape)))
+        d = torch.from_numpy(rng.uniform(-1, 1, tuple(delta0.shape))).to(delta0.device)
         ad = float(torch.sum(g * d))
         fd = (J_at(delta0 + eps * d, re, th) - J_at(delta0 - eps * d, re, th)) / (2 * eps)
         rel = abs(ad - fd) / abs(fd)
