@@ -209,14 +209,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     constexpr int NZX = (DEN ? 6 : 0) + (HEAT ? 1 : 0);
     __shared__ double zx[NZX > 0 ? NZX : 1][KCMAX + 2];
     __shared__ double hgx_s[HEAT ? 66 : 1], hgy_s[HEAT ? ROWS : 1];
-    // Stage 3: the column's min(ddx, ddy) and the lane's running dt extrema
-    // live in LDS rather than in registers across the z loop.  Held in
-    // registers they are spilled, and a scratch reload after a plane's stores
-    // waits for all of them (vector memory operations complete in issue
-    // order); an LDS access does not.
+    // Stage 3: the column's min(ddx, ddy) and each flux row's running dt
+    // extrema live in LDS rather than in registers across the z loop.  Held
+    // in registers they are spilled, and a scratch reload after a plane's
+    // stores waits for all of them (vector memory operations complete in
+    // issue order); an LDS access does not.  The extrema are kept per row
+    // (wave), merged by a wave reduction on the rare planes where some cell
+    // of the row passes the time-step screen.
     __shared__ double ddxy_s[STAGE == 2 ? ROWS : 1][66];
     constexpr int NPM = MAG ? NPART : NPART - 1;  // extrema in use (no resistive limit without B)
-    __shared__ double pm_s[STAGE == 2 ? NPM : 1][STAGE == 2 ? TY : 1][64];
+    __shared__ double pmw_s[STAGE == 2 ? NPM : 1][STAGE == 2 ? TY : 1];
 
     // stage 3 screen thresholds from the previous reduction (DevScal); read
     // from LDS where they are used, so they hold no register across the loop
@@ -319,8 +321,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     const double dyy = M.dyy[lj], d2yy = M.d2yy[lj];
     if (STAGE == 2 && heavy) {
         ddxy_s[r][c] = dmin(M.ddx[li], M.ddy[lj]);
-        pm_s[0][r - 1][ln] = 0.0;
-        for (int q = 1; q < NPM; ++q) pm_s[q][r - 1][ln] = 1e300;
+        if (ln == 0) {
+            pmw_s[0][r - 1] = 0.0;
+            for (int q = 1; q < NPM; ++q) pmw_s[q][r - 1] = 1e300;
+        }
     }
 
     Col cm = derive(load_raw<MAG>(a.xin, g.pp, k0 - 1, voff));
@@ -719,14 +723,25 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 const double rk = zm[3][kl];
                 // the exact limits on output cells that pass the screen (all
                 // with a.screen = 0); the others leave the extrema as they are
+                bool hit = false;
                 if (outp && !bplane &&
                     (!a.screen || dt_screen<MAG>(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT],
                                                  bxv, byv, bzv, dd, rk, scr_s))) {
                     dt_terms<MAG>(C, xv[QRU], xv[QRV], xv[QRW], xv[QRO], xv[QTT], bxv, byv, bzv,
                                   dd, rk, q);
-                    pm_s[0][r - 1][ln] = dmax(pm_s[0][r - 1][ln], q[0]);
-                    for (int v = 1; v < NPM; ++v)
-                        pm_s[v][r - 1][ln] = dmin(pm_s[v][r - 1][ln], q[v]);
+                    hit = true;
+                }
+                // the row's extrema, on planes where some lane evaluated
+                // (lanes that did not hold the identities)
+                if (__ballot(hit)) {
+                    const double v0 = wave_max(q[0]);
+                    double vm[NPM];
+                    for (int v = 1; v < NPM; ++v) vm[v] = wave_min(q[v]);
+                    if (ln == 0) {
+                        pmw_s[0][r - 1] = dmax(pmw_s[0][r - 1], v0);
+                        for (int v = 1; v < NPM; ++v)
+                            pmw_s[v][r - 1] = dmin(pmw_s[v][r - 1], vm[v]);
+                    }
                 }
             }
         }
@@ -744,7 +759,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
 
     if (STAGE == 2 && a.want_dt) {
         if (heavy)
-            for (int q = 0; q < NPM; ++q) pm[q] = pm_s[q][r - 1][ln];
+            for (int q = 0; q < NPM; ++q) pm[q] = pmw_s[q][r - 1];
         block_partials<NW>(pm, red, a.partial, t);
     }
 }
