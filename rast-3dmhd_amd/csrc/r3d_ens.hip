@@ -47,7 +47,7 @@ __device__ __forceinline__ StageArgs member_args(const EnsMember& e, int cur, in
     a.stage = stage;
     a.zmode = zmode;
     a.want_dt = want_dt;
-    a.screen = 0;
+    a.screen = stage == 2 && want_dt;  // the time-step screen, as in a single run
     a.ntx = e.ntx;
     a.nty = e.nty;
     a.nkc = e.nkc;
@@ -92,11 +92,30 @@ __global__ __launch_bounds__(DT_THREADS) void k_dt_ens(const EnsMember* __restri
     dt_block(e.g, e.x[cur], e.C, e.M, e.partial, blockIdx.x, gridDim.x);
 }
 
-// grid = (members); fused: partials of the last stage, else of k_dt_ens
+// grid = (members); fused: the screened partials of the last stage, else
+// those of k_dt_ens
 __global__ __launch_bounds__(1024) void k_finish_ens(const EnsMember* __restrict__ mem,
                                                      int fused, int advance) {
     const EnsMember& e = mem[blockIdx.x];
-    finish_block(e.scal, e.partial, fused ? e.nblocks + 2 * e.nbb : e.npart_dt, e.C, 1, advance);
+    finish_block(e.scal, e.partial, fused ? e.nblocks + 2 * e.nbb : e.npart_dt, e.C, 1, advance,
+                 fused);
+}
+
+// The exact dt pass of the members whose screened extrema were not proven
+// (DevScal::dt_redo); the blocks of every other member return at once.
+// grid = (ENS_DT_BLOCKS, members)
+__global__ __launch_bounds__(DT_THREADS) void k_dt_redo_ens(const EnsMember* __restrict__ mem,
+                                                            int cur) {
+    const EnsMember& e = mem[blockIdx.y];
+    if (!e.scal->dt_redo) return;
+    dt_block(e.g, e.x[cur], e.C, e.M, e.partial, blockIdx.x, gridDim.x);
+}
+// grid = (members): all threads read the flag before the reduction's first
+// barrier; thread 0 clears it after the last
+__global__ __launch_bounds__(1024) void k_finish_redo_ens(const EnsMember* __restrict__ mem) {
+    const EnsMember& e = mem[blockIdx.x];
+    if (!e.scal->dt_redo) return;
+    finish_block(e.scal, e.partial, e.npart_dt, e.C, 1, 0, 0);
 }
 
 struct Tiling {
@@ -200,6 +219,11 @@ int enqueue_step(r3d_ens_ctx* e, int cur) {
         cur ^= 1;
     }
     hipLaunchKernelGGL(k_finish_ens, dim3(n), dim3(1024), 0, e->stream, e->d_desc, 1, 1);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_dt_redo_ens, dim3(ENS_DT_BLOCKS, n), dim3(DT_THREADS), 0, e->stream,
+                       e->d_desc, cur);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_finish_redo_ens, dim3(n), dim3(1024), 0, e->stream, e->d_desc);
     HIPCHK(hipGetLastError());
     return R3D_OK;
 }
