@@ -347,6 +347,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // lane + 1: no per-lane row index has to stay live in that loop
     const int r = HEAVY ? wv : r_all;
     const int c = HEAVY ? ln + 1 : c_all;
+    // Every load issued before the loop has landed before it starts.  The
+    // compiler's wait counts at the loop head merge the entry path with the
+    // back edge: a load still in flight on entry would otherwise make every
+    // plane wait, right after its barrier, for all of the previous plane's
+    // stores (vector memory operations retire in issue order).
+    __builtin_amdgcn_s_waitcnt(0);
     for (int k = k0; k <= k1; ++k) {
         const int buf = k & 1;
         // Keep the lane offsets opaque to loop-invariant code motion: every
@@ -354,10 +360,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
         // instead of one hoisted 64-bit per-lane address per field.
         asm volatile("" : "+v"(voff));
         asm volatile("" : "+v"(soff));
-        const Raw rnn = load_raw<MAG>(a.xin, g.pp, min(k + 2, g.npz + 1), voff);
+        // This plane's Z is issued before the next-but-one X plane, in the
+        // order the flux blocks consume it (FR first): the first RK update
+        // then waits for the Z of RO only, not for the X loads it does not use.
         double zc[NF];
-        if constexpr (STAGE > 0 && HEAVY)
-            for (int f = 0; f < NF; ++f) zc[f] = ldp_stream(a.z.q[f], g.pp, k, voff);
+        if constexpr (STAGE > 0 && HEAVY) {
+            zc[QRO] = ldp_stream(a.z.q[QRO], g.pp, k, voff);
+            zc[QRU] = ldp_stream(a.z.q[QRU], g.pp, k, voff);
+            zc[QRV] = ldp_stream(a.z.q[QRV], g.pp, k, voff);
+            zc[QRW] = ldp_stream(a.z.q[QRW], g.pp, k, voff);
+            zc[QTT] = ldp_stream(a.z.q[QTT], g.pp, k, voff);
+            for (int f = QBX; f < NF; ++f) zc[f] = ldp_stream(a.z.q[f], g.pp, k, voff);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const Raw rnn = load_raw<MAG>(a.xin, g.pp, min(k + 2, g.npz + 1), voff);
         const Col cp = derive(rn);
         const int kl = k - k0;
         const double dzz = zm[0][kl], d2zz = zm[1][kl];

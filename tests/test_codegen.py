@@ -91,3 +91,27 @@ def test_hydro_stage_kernel_memory_path(asm, ty, stage, zmode):
 def test_other_stage_kernels_have_no_flat_access(asm, stage, zmode, mag, heat, den):
     lines = kernel_body(asm, 5 if mag else 8, stage, zmode, mag=mag, heat=heat, den=den)
     assert not [l for l in lines if re.match(r"\s+flat_", l)], "FLAT memory instruction"
+
+
+@pytest.mark.parametrize("ty,mag", [(8, False), (9, False), (5, True)], ids=["hydro8", "hydro9", "mhd"])
+@pytest.mark.parametrize("stage,zmode", FULL_STEP)
+def test_no_store_drain_after_the_plane_barrier(asm, ty, mag, stage, zmode):
+    """Between a plane's barrier and its first RK-update store, no wait may
+    leave all of the plane's own loads in flight: such a wait (vmcnt(N) with
+    N >= the loads issued this plane) only drains the previous plane's stores.
+    It appears when a load issued before the z loop is still pending at the
+    loop head."""
+    loop = largest_loop(kernel_body(asm, ty, stage, zmode, mag=mag))
+    bar = next(n for n, l in enumerate(loop) if l.startswith("s_barrier"))
+    # the loop body in execution order from the barrier round to the barrier
+    # (the compiler may place the post-barrier blocks first)
+    cyc = loop[bar + 1:] + loop[:bar + 1]
+    stores = [n for n, l in enumerate(cyc) if l.startswith("global_store")]
+    # loads issued after the plane's last store: in flight at the barrier,
+    # behind the stores
+    nload = len([l for l in cyc[stores[-1]:] if l.startswith("global_load")])
+    drains = [l for l in cyc[:stores[0]]
+              if l.startswith("s_waitcnt") and "vmcnt" in l
+              and int(re.search(r"vmcnt\((\d+)\)", l).group(1)) >= nload]
+    assert nload >= 5, "plane loads not found before the barrier"
+    assert not drains, f"store drain after the barrier: {drains} ({nload} loads in flight)"
