@@ -280,8 +280,9 @@ def run_checkpoint(a):
 
 
 def measure_ensemble(pkg, nmem, steps, warmup):
-    """The batched ensemble (autotuned tiles, graph replay) and the same
-    members stepped one after another through the plain path."""
+    """The batched ensemble (autotuned tiles; graph replay, then the same
+    steps as one persistent cooperative launch) and the same members stepped
+    one after another through the plain path."""
     members = pkg.ensemble_members(nmem)
     runs = [pkg.Run(m) for m in members]
     cells = sum(r.npx * r.npy * r.npz for r in runs)
@@ -296,9 +297,18 @@ def measure_ensemble(pkg, nmem, steps, warmup):
     ens.step(steps)
     ens.sync()
     wall = time.perf_counter() - t0
+    # the same steps as one persistent cooperative launch
+    ens.set_persistent(True)
+    ens.step(1)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.sync()
+    wall_p = time.perf_counter() - t0
     mem = ens.memory_bytes()
     ens.close()
-    log(f"[bench] ensemble {wall / steps * 1e3:.3f} ms/step; sequential reference next")
+    log(f"[bench] ensemble {wall / steps * 1e3:.3f} ms/step (graph replay), "
+        f"{wall_p / steps * 1e3:.3f} ms/step (one persistent launch); sequential reference next")
     seq = 0.0
     for r in runs:
         sim = r.make_sim()
@@ -311,7 +321,7 @@ def measure_ensemble(pkg, nmem, steps, warmup):
         sim.close()
         r.close()
     return {"members": len(runs), "cells": cells, "wall": wall, "seq": seq, "tiles": (ty, kc),
-            "mem": mem}
+            "mem": mem, "wall_persistent": wall_p}
 
 
 def run_ensemble(a):
@@ -332,7 +342,9 @@ def run_ensemble(a):
                                "Re in [1,4], theta in [0.1,0.3], virtual 1x2 each",
                    "global_cells": cells, "tiles": [ty, kc], "graph": True,
                    "device_bytes": mem, "sequential_value": seq_value,
-                   "speedup_vs_sequential": value / seq_value},
+                   "speedup_vs_sequential": value / seq_value,
+                   "persistent_value": cells * 3 * a.steps / e["wall_persistent"],
+                   "persistent_speedup_vs_sequential": seq / e["wall_persistent"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "note": "whole-step rate (all kernels of the step), not one kernel"},
@@ -529,6 +541,8 @@ def secondary(pkg):
                        "value": ev, "unit": "cell-updates/s",
                        "sequential_value": e["cells"] * 3 * 20 / e["seq"],
                        "speedup_vs_sequential": e["seq"] / e["wall"],
+                       "persistent_value": e["cells"] * 3 * 20 / e["wall_persistent"],
+                       "persistent_speedup_vs_sequential": e["seq"] / e["wall_persistent"],
                        "frac": BYTES_PER_CELL_STAGE * ev / (HBM_PEAK_GBS * 1e9)}
     return out
 

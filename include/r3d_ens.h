@@ -54,6 +54,13 @@ int r3d_ens_autotune(r3d_ens_ctx* e, int trial_steps, double* best_ms_per_step);
 /* enable != 0: steps replay captured HIP graphs (default); 0: plain launches. */
 int r3d_ens_set_graph(r3d_ens_ctx* e, int enable);
 
+/* enable != 0: each r3d_ens_step call runs all its steps in one persistent
+ * cooperative launch (every block resident, grid-wide barriers between the
+ * phases of a stage and of a step) instead of 7 launches per step; results
+ * are bit-identical.  R3D_EUNSUP from r3d_ens_step where the device has no
+ * cooperative launch. */
+int r3d_ens_set_persistent(r3d_ens_ctx* e, int enable);
+
 /* Advance every member by nsteps full RK3 steps (asynchronous). */
 int r3d_ens_step(r3d_ens_ctx* e, int nsteps);
 int r3d_ens_sync(r3d_ens_ctx* e);

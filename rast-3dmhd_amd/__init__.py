@@ -116,6 +116,7 @@ def load():
         "r3d_ens_get_tiles": [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip)],
         "r3d_ens_autotune": [vp, ip, ctypes.POINTER(ctypes.c_double)],
         "r3d_ens_set_graph": [vp, ip],
+        "r3d_ens_set_persistent": [vp, ip],
         "r3d_ens_step": [vp, ip],
         "r3d_ens_sync": [vp],
         "r3d_ens_memory": [vp, ctypes.POINTER(ctypes.c_ulonglong)],
@@ -359,6 +360,10 @@ class Ensemble:
 
     def set_graph(self, on):
         _check(load().r3d_ens_set_graph(self.h, 1 if on else 0))
+
+    def set_persistent(self, on):
+        """One cooperative launch per step() call for all its steps."""
+        _check(load().r3d_ens_set_persistent(self.h, 1 if on else 0))
 
     def scalars(self, m):
         s = Scalars()

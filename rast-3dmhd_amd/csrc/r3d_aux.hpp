@@ -22,14 +22,16 @@ namespace R3D_KNS {
 constexpr int BOUND_THREADS = 256;
 constexpr int DT_THREADS = 256;
 
-// Block bx of nbb (per side) of the boundary pass; side 0 = top, 1 = bottom.
-// Its dt partial goes to slot partial_base + side * nbb + bx.
+// Block bx of nbb (per side) of the boundary pass, NT threads a block; side
+// 0 = top, 1 = bottom.  Its dt partial goes to slot partial_base + side * nbb
+// + bx.
+template <int NT = BOUND_THREADS>
 __device__ __forceinline__ void bound_block(const StageArgs& a, const Consts& C, const Metrics& M,
                                             int partial_base, int bx, int nbb, int side) {
-    __shared__ double red[BOUND_THREADS / 64][NPART];
+    __shared__ double red[NT / 64][NPART];
     const Geo g = a.g;
     const int W = g.npx + 2, H = g.npy + 2;
-    const int idx = bx * BOUND_THREADS + threadIdx.x;
+    const int idx = bx * NT + threadIdx.x;
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
     const double dt = a.scal->dt;
     const double cz = (a.stage < 2) ? C.zeta[a.stage] * dt : 0.0;
@@ -119,17 +121,18 @@ __device__ __forceinline__ void bound_block(const StageArgs& a, const Consts& C,
         }
     }
     if (a.stage == 2 && a.want_dt)
-        block_partials<BOUND_THREADS / 64>(pm, red, a.partial, partial_base + side * nbb + bx);
+        block_partials<NT / 64>(pm, red, a.partial, partial_base + side * nbb + bx);
 }
 
 // dt limits of the state `s` (reference src/step.cpp:121-209): block bx of
-// nbx, one partial per block.
+// nbx (NT threads each), one partial per block.
+template <int NT = DT_THREADS>
 __device__ __forceinline__ void dt_block(const Geo& g, const State& s, const Consts& C,
                                          const Metrics& M, double* partial, int bx, int nbx) {
-    __shared__ double red[DT_THREADS / 64][NPART];
+    __shared__ double red[NT / 64][NPART];
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
     const long total = (long)g.npx * g.npy * g.npz;
-    for (long q = (long)bx * DT_THREADS + threadIdx.x; q < total; q += (long)nbx * DT_THREADS) {
+    for (long q = (long)bx * NT + threadIdx.x; q < total; q += (long)nbx * NT) {
         const int i = 1 + (int)(q % g.npx);
         const long rest = q / g.npx;
         const int j = 1 + (int)(rest % g.npy);
@@ -143,7 +146,7 @@ __device__ __forceinline__ void dt_block(const Geo& g, const State& s, const Con
             dt_terms<false>(C, s.q[QRU][n], s.q[QRV][n], s.q[QRW][n], s.q[QRO][n], s.q[QTT][n],
                             0.0, 0.0, 0.0, dd, M.rkdt[k], pm);
     }
-    block_partials<DT_THREADS / 64>(pm, red, partial, bx);
+    block_partials<NT / 64>(pm, red, partial, bx);
 }
 
 
@@ -171,7 +174,7 @@ __device__ __forceinline__ void halo_thread(const Geo& g, const State& x, int nf
     for (int f = 0; f < nf; ++f) x.q[f][d] = x.q[f][s];
 }
 
-// One block of 1024 threads.  reduce: fold partial[0..npart) into dt_next /
+// One block of NT threads.  reduce: fold partial[0..npart) into dt_next /
 // umach_next (dt = sff * min of the limits, src/step.cpp:224-229; umach =
 // sqrt(max), src/step.cpp:141).  advance: the end-of-step counters
 // (src/step.cpp:254-256).
@@ -181,13 +184,14 @@ __device__ __forceinline__ void halo_thread(const Geo& g, const State& x, int nf
 // um_prev*scr_dn): a cell with a smaller limit or a larger ogamma*UU/T would
 // have passed the screen.  Otherwise dt_redo is set and dt_next is left to
 // the exact pass.
+template <int NT = 1024>
 __device__ __forceinline__ void finish_block(DevScal* sc, const double* partial, int npart,
                                              const Consts& C, int reduce, int advance,
                                              int screened = 0) {
-    __shared__ double red[16][NPART];
+    __shared__ double red[NT / 64][NPART];
     double pm[NPART] = {0.0, 1e300, 1e300, 1e300, 1e300};
     if (reduce)
-        for (int q = threadIdx.x; q < npart; q += 1024) {
+        for (int q = threadIdx.x; q < npart; q += NT) {
             pm[0] = dmax(pm[0], partial[(long)q * NPART]);
             for (int v = 1; v < NPART; ++v) pm[v] = dmin(pm[v], partial[(long)q * NPART + v]);
         }
@@ -207,7 +211,7 @@ __device__ __forceinline__ void finish_block(DevScal* sc, const double* partial,
         if (reduce) {
             double o[NPART];
             for (int q = 0; q < NPART; ++q) o[q] = red[0][q];
-            for (int w = 1; w < 16; ++w) {
+            for (int w = 1; w < NT / 64; ++w) {
                 o[0] = dmax(o[0], red[w][0]);
                 for (int q = 1; q < NPART; ++q) o[q] = dmin(o[q], red[w][q]);
             }

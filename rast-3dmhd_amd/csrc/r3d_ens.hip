@@ -8,6 +8,7 @@
 // way one large run does.  The boundary pass, the dt reductions and the
 // end-of-step bookkeeping are likewise one launch each, indexed by member.
 // A step is captured in a HIP graph per X parity and replayed.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +30,7 @@ struct EnsMember {
     double* partial;
     DevScal* scal;
     int ntx, nty, nkc, kc, nblocks, nbb, tile0, npart_dt;
+    int bnd0, nbbp;  // persistent launch: first boundary unit, boundary blocks per side
     Consts C;
     Metrics M;
 };
@@ -62,6 +64,7 @@ __device__ __forceinline__ StageArgs member_args(const EnsMember& e, int cur, in
 template <int TY, int STAGE, bool MAG>
 __global__ __launch_bounds__((TY + 3) * 64) void k_stage_ens(const EnsMember* __restrict__ mem,
                                                              int nmem, int ntiles, int cur) {
+    __shared__ StageLds<TY, STAGE, MAG, false, false> lds;
     const int T = logical_block();
     if (T >= ntiles) return;
     int lo = 0, hi = nmem - 1;  // last member whose first tile is <= T
@@ -73,7 +76,120 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage_ens(const EnsMember* __
     const EnsMember& e = mem[lo];
     constexpr int ZMODE = STAGE < 2 ? 0 : 2;
     const StageArgs a = member_args(e, cur, STAGE, ZMODE, STAGE == 2);
-    stage_tile<TY, STAGE, ZMODE, MAG>(a, e.C, e.M, T - e.tile0);
+    stage_tile<TY, STAGE, ZMODE, MAG>(a, e.C, e.M, T - e.tile0, lds);
+}
+
+// Last member m with key(mem[m]) <= v (keys ascending from mem[0] = 0).
+template <class Key>
+__device__ __forceinline__ int find_member(const EnsMember* mem, int nmem, int v, Key key) {
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(mem[mid]) <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The persistent form: nsteps full steps of every member in one cooperative
+// launch of NT-thread blocks, all resident (at most one per CU at this LDS
+// size).  Each phase is a loop of its blocks over work units, followed by a
+// grid-wide barrier (cooperative groups: every block writes its L2 back and
+// invalidates it, so a phase sees the previous phase's stores on every XCD):
+//   per stage: the stage tiles, then the boundary pass;
+//   per step: each member's dt reduction with the screen's proof, then, only
+//   when some member's proof failed, the exact pass and its reduction for
+//   those members.
+// Stage tiles are taken dynamically: the global tile order is cut into eight
+// contiguous ranges, one per XCD (blocks b with b % 8 = x, as the per-stage
+// launches place them), and a block takes the next tile of its range from a
+// counter (work[x]).  Counters are never reset during the launch: each block
+// takes exactly one tile past the end of its range per stage, so the next
+// stage's range starts per_xcd further on.  work[8] is set by a member whose
+// proof failed and read by every block after the barrier that follows, so
+// all blocks agree on whether the exact pass runs; block 0 clears it one
+// barrier later.  Every block runs the same sequence of barriers and the work
+// loops contain none.  Results are those of the per-stage launches bit for
+// bit (the same tile and cell code; the min / max reductions do not depend on
+// how the partials are grouped).  work[0..8] must be zero at launch.
+template <int TY, bool MAG>
+__global__ __launch_bounds__((TY + 3) * 64) void k_ens_run(const EnsMember* __restrict__ mem,
+                                                           int nmem, int ntiles, int nbunits,
+                                                           int cur, int nsteps, int* work) {
+    constexpr int NT = (TY + 3) * 64;
+    __shared__ union {
+        StageLds<TY, 0, MAG, false, false> s0;
+        StageLds<TY, 1, MAG, false, false> s1;
+        StageLds<TY, 2, MAG, false, false> s2;
+    } lds;
+    __shared__ int grab;
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int nblk = gridDim.x;  // a multiple of 8
+    const int xcd = blockIdx.x & 7, per_xcd = nblk >> 3;
+    const int tiles_xcd = (ntiles + 7) >> 3;
+    const int n_x = max(0, min(tiles_xcd, ntiles - xcd * tiles_xcd));  // this XCD's tiles
+    int base = 0;  // counter value of this XCD's first tile in the current stage
+    for (int st = 0; st < nsteps; ++st) {
+        for (int s = 0; s < 3; ++s) {
+            if (st + s > 0 && blockIdx.x == 0 && threadIdx.x == 0 && s == 0)
+                work[8] = 0;  // every block read it before the last barrier
+            for (;;) {
+                if (threadIdx.x == 0) grab = atomicAdd(&work[xcd], 1) - base;
+                __syncthreads();
+                const int i = grab;
+                __syncthreads();  // grab is rewritten by the next take
+                if (i >= n_x) break;
+                const int T = xcd * tiles_xcd + i;
+                const EnsMember& e =
+                    mem[find_member(mem, nmem, T, [](const EnsMember& x) { return x.tile0; })];
+                const StageArgs a = member_args(e, cur, s, s < 2 ? 0 : 2, s == 2);
+                if (s == 0) stage_tile<TY, 0, 0, MAG>(a, e.C, e.M, T - e.tile0, lds.s0);
+                else if (s == 1) stage_tile<TY, 1, 0, MAG>(a, e.C, e.M, T - e.tile0, lds.s1);
+                else stage_tile<TY, 2, 2, MAG>(a, e.C, e.M, T - e.tile0, lds.s2);
+                __syncthreads();  // the next tile reuses the LDS
+            }
+            base += n_x + per_xcd;
+            grid.sync();
+            for (int u = blockIdx.x; u < nbunits; u += nblk) {
+                const EnsMember& e =
+                    mem[find_member(mem, nmem, u, [](const EnsMember& x) { return x.bnd0; })];
+                const int l = u - e.bnd0;
+                const StageArgs a = member_args(e, cur, s, s < 2 ? 0 : 2, s == 2);
+                bound_block<NT>(a, e.C, e.M, e.nblocks, l % e.nbbp, e.nbbp, l / e.nbbp);
+                __syncthreads();
+            }
+            grid.sync();
+            cur ^= 1;
+        }
+        for (int m = blockIdx.x; m < nmem; m += nblk) {
+            const EnsMember& e = mem[m];
+            finish_block<NT>(e.scal, e.partial, e.nblocks + 2 * e.nbbp, e.C, 1, 1, 1);
+            if (threadIdx.x == 0 && e.scal->dt_redo) work[8] = 1;
+            __syncthreads();
+        }
+        grid.sync();
+        if (work[8]) {  // the same value in every block
+            for (int u = blockIdx.x; u < nmem * ENS_DT_BLOCKS; u += nblk) {
+                const EnsMember& e = mem[u / ENS_DT_BLOCKS];
+                if (e.scal->dt_redo) {
+                    dt_block<NT>(e.g, e.x[cur], e.C, e.M, e.partial, u % ENS_DT_BLOCKS,
+                                 ENS_DT_BLOCKS);
+                    __syncthreads();
+                }
+            }
+            grid.sync();
+            // all threads read the flag before the reduction's barrier;
+            // thread 0 clears it after
+            for (int m = blockIdx.x; m < nmem; m += nblk) {
+                const EnsMember& e = mem[m];
+                if (e.scal->dt_redo) {
+                    finish_block<NT>(e.scal, e.partial, e.npart_dt, e.C, 1, 0, 0);
+                    __syncthreads();
+                }
+            }
+            grid.sync();
+        }
+    }
 }
 
 // grid = (max boundary blocks, 2 sides, members)
@@ -129,12 +245,15 @@ struct r3d_ens_ctx {
     std::vector<double*> part;     // per-member partial buffers sized for any tiling
     std::vector<EnsMember> desc;
     EnsMember* d_desc = nullptr;
+    int* d_work = nullptr;  // persistent launch: tile counters per XCD, exact-pass flag
     hipStream_t stream = nullptr;
     int device = 0, lmag = 0;
     Tiling tile{8, 0};
     int ntiles = 0, grid = 0, nbb_max = 0;
     int cur = 0;
     bool use_graph = true;
+    bool persistent = false;  // one cooperative launch per r3d_ens_step call
+    int nbunits = 0;          // persistent launch: boundary units of all members
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     unsigned long long bytes = 0;
 };
@@ -154,9 +273,13 @@ void drop_graphs(r3d_ens_ctx* e) {
         }
 }
 
+// Threads of a persistent-launch block at the current tiling.
+int persistent_threads(const r3d_ens_ctx* e) { return (e->tile.ty + 3) * 64; }
+
 // Rebuild the member table for the current tiling and parity.
 int build_table(r3d_ens_ctx* e) {
-    int tile0 = 0;
+    int tile0 = 0, bnd0 = 0;
+    const int npt = persistent_threads(e);
     e->nbb_max = 0;
     for (size_t m = 0; m < e->mem.size(); ++m) {
         r3d_hip_ctx* c = e->mem[m];
@@ -175,12 +298,16 @@ int build_table(r3d_ens_ctx* e) {
         d.nbb = c->nbb;
         d.tile0 = tile0;
         d.npart_dt = ENS_DT_BLOCKS;
+        d.nbbp = ((c->cfg.npx + 2) * (c->cfg.npy + 2) + npt - 1) / npt;
+        d.bnd0 = bnd0;
+        bnd0 += 2 * d.nbbp;
         d.C = c->C;
         d.M = c->M;
         tile0 += d.nblocks;
         e->nbb_max = std::max(e->nbb_max, d.nbb);
     }
     e->ntiles = tile0;
+    e->nbunits = bnd0;
     e->grid = (tile0 + 7) / 8 * 8;
     HIPCHK(hipMemcpyAsync(e->d_desc, e->desc.data(), e->desc.size() * sizeof(EnsMember),
                           hipMemcpyHostToDevice, e->stream));
@@ -226,6 +353,38 @@ int enqueue_step(r3d_ens_ctx* e, int cur) {
     hipLaunchKernelGGL(k_finish_redo_ens, dim3(n), dim3(1024), 0, e->stream, e->d_desc);
     HIPCHK(hipGetLastError());
     return R3D_OK;
+}
+
+// nsteps full steps in one cooperative launch (k_ens_run): as many blocks as
+// can be resident at once (a multiple of 8, at most one per tile), each
+// looping over its share of every phase.
+template <int TY, bool MAG>
+int launch_persistent_ty(r3d_ens_ctx* e, int nsteps) {
+    constexpr int NT = (TY + 3) * 64;
+    const void* fn = reinterpret_cast<const void*>(&k_ens_run<TY, MAG>);
+    int coop = 0, ncu = 0, per_cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, e->device));
+    if (!coop) return set_err(R3D_EUNSUP, "device %d has no cooperative launch", e->device);
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, e->device));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, 0));
+    int nblk = std::min(per_cu * ncu, (e->ntiles + 7) / 8 * 8) / 8 * 8;
+    if (nblk < 8)
+        return set_err(R3D_EUNSUP, "persistent ensemble: %d resident blocks of %d threads", nblk,
+                       NT);
+    const EnsMember* d = e->d_desc;
+    int nmem = (int)e->mem.size(), ntiles = e->ntiles, nbu = e->nbunits, cur = e->cur, ns = nsteps;
+    int* work = e->d_work;
+    HIPCHK(hipMemsetAsync(work, 0, 16 * sizeof(int), e->stream));
+    void* args[] = {(void*)&d, &nmem, &ntiles, &nbu, &cur, &ns, (void*)&work};
+    HIPCHK(hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(NT), args, 0, e->stream));
+    return R3D_OK;
+}
+
+int launch_persistent(r3d_ens_ctx* e, int nsteps) {
+    if (e->lmag) return launch_persistent_ty<TILE_Y_MAG, true>(e, nsteps);
+    if (e->tile.ty == 4) return launch_persistent_ty<4, false>(e, nsteps);
+    if (e->tile.ty == 8) return launch_persistent_ty<8, false>(e, nsteps);
+    return launch_persistent_ty<9, false>(e, nsteps);
 }
 
 int enqueue_dt(r3d_ens_ctx* e) {
@@ -326,6 +485,8 @@ int r3d_ens_create(int nmembers, const r3d_hip_cfg* cfgs, r3d_ens_ctx** out) {
     e->desc.resize(nmembers);
     if (hipMalloc((void**)&e->d_desc, nmembers * sizeof(EnsMember)) != hipSuccess)
         return fail(set_err(R3D_ENOMEM, "member table"));
+    if (hipMalloc((void**)&e->d_work, 16 * sizeof(int)) != hipSuccess)
+        return fail(set_err(R3D_ENOMEM, "persistent work counters"));
     if (hipSetDevice(e->device) != hipSuccess ||
         hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(R3D_EHIP, "ensemble stream"));
@@ -340,6 +501,7 @@ int r3d_ens_destroy(r3d_ens_ctx* e) {
     for (double* p : e->part) (void)hipFree(p);
     for (r3d_hip_ctx* c : e->mem) r3d_hip_destroy(c);
     if (e->d_desc) (void)hipFree(e->d_desc);
+    if (e->d_work) (void)hipFree(e->d_work);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return R3D_OK;
@@ -421,6 +583,13 @@ int r3d_ens_set_graph(r3d_ens_ctx* e, int enable) {
     return R3D_OK;
 }
 
+int r3d_ens_set_persistent(r3d_ens_ctx* e, int enable) {
+    if (!e) return set_err(R3D_EINVAL, "null ensemble");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->persistent = enable != 0;
+    return R3D_OK;
+}
+
 int r3d_ens_step(r3d_ens_ctx* e, int nsteps) {
     if (!e) return set_err(R3D_EINVAL, "null ensemble");
     if (nsteps < 0) return set_err(R3D_EINVAL, "nsteps < 0");
@@ -432,6 +601,12 @@ int r3d_ens_step(r3d_ens_ctx* e, int nsteps) {
     // members may have used their own streams (uploads, scalars)
     for (r3d_hip_ctx* c : e->mem) HIPCHK(hipStreamSynchronize(c->stream));
     if (!all_next_valid(e) && (rc = enqueue_dt(e))) return rc;
+    if (e->persistent) {
+        if ((rc = launch_persistent(e, nsteps))) return rc;
+        e->cur ^= nsteps & 1;  // three stages per step flip the X parity
+        sync_members(e, true);
+        return R3D_OK;
+    }
     for (int st = 0; st < nsteps; ++st) {
         if (e->use_graph) {
             if (!e->graph[e->cur] && (rc = capture(e, e->cur))) return rc;

@@ -177,38 +177,30 @@ __device__ __forceinline__ void block_partials(double pm[NPART], double (*red)[N
     }
 }
 
-// One tile t (0 <= t < a.nblocks) of one RK stage; the caller maps blocks to
-// tiles.  ZMODE: 0 = Z' is the next stage's mix, 1 = Z keeps the saved state
-// (a step truncated after stage 1), 2 = Z is not written (last stage).
-// HEAT: the ITCON=2 embedded heat loss term is added to FT; DEN: density
-// diffusion is added to FR.  Separate instantiations, so that the default
-// kernels carry none of their registers or LDS.
-template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false, bool DEN = false>
-__device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, const Metrics& M,
-                                           const int t) {
-    constexpr int ROWS = TY + 2;
-    constexpr int NW = TY + 3;
-    constexpr int NF = MAG ? 8 : 5;  // primaries per state
-    constexpr int SH_RO = MAG ? NSH_MAG : NSH;  // DEN only: raw RO of the plane
-    __shared__ double sh[2][(MAG ? NSH_MAG : NSH) + (DEN ? 1 : 0)][ROWS][66];
-    __shared__ double red[NW][NPART];
+// LDS of one stage tile.  The kernel that runs the tile declares it, so that
+// a persistent kernel can run the three stages in one allocation (a union).
+template <int TY, int STAGE, bool MAG, bool HEAT, bool DEN>
+struct StageLds {
+    static constexpr int ROWS = TY + 2, NW = TY + 3;
+    static constexpr int KCMAX = 128;
+    static constexpr int NZX = (DEN ? 6 : 0) + (HEAT ? 1 : 0);
+    static constexpr int NPM = MAG ? NPART : NPART - 1;  // extrema in use (no resistive limit without B)
+    double sh[2][(MAG ? NSH_MAG : NSH) + (DEN ? 1 : 0)][ROWS][66];
+    double red[NW][NPART];
     // Per-plane z metrics of this block's chunk, staged once: they are
     // uniform per plane, and fetching them inside the z loop would put a
     // global-memory round trip on every plane.
-    constexpr int KCMAX = 128;
     // dzz, d2zz, ddz, the conductivity (rkapa; 1 + rkapa with LREM, whose
     // diffusion does not read it), dkapa
-    __shared__ double zm[5][KCMAX];
+    double zm[5][KCMAX];
     // DEN / HEAT per-plane values, staged with the metrics for the same
     // reason: ROM at k-1..k+1 (row 0, offset by one plane), DH*ORE*WWZ,
     // DV*ORE*WWZ, RLAX*TFAC*CORRECT, then the heat loss z Gaussian; and the heat loss x / y
     // Gaussians of the tile's columns and rows, so that they hold no
     // registers across the z loop
     // and with LREM TTM at k-1..k+1 (row ZX_DEN + 4, offset by one plane) and HRAD
-    constexpr int ZX_DEN = 0, ZX_HEAT = DEN ? 6 : 0;
-    constexpr int NZX = (DEN ? 6 : 0) + (HEAT ? 1 : 0);
-    __shared__ double zx[NZX > 0 ? NZX : 1][KCMAX + 2];
-    __shared__ double hgx_s[HEAT ? 66 : 1], hgy_s[HEAT ? ROWS : 1];
+    double zx[NZX > 0 ? NZX : 1][KCMAX + 2];
+    double hgx_s[HEAT ? 66 : 1], hgy_s[HEAT ? ROWS : 1];
     // Stage 3: the column's min(ddx, ddy) and each flux row's running dt
     // extrema live in LDS rather than in registers across the z loop.  Held
     // in registers they are spilled, and a scratch reload after a plane's
@@ -216,13 +208,41 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     // issue order); an LDS access does not.  The extrema are kept per row
     // (wave), merged by a wave reduction on the rare planes where some cell
     // of the row passes the time-step screen.
-    __shared__ double ddxy_s[STAGE == 2 ? ROWS : 1][66];
-    constexpr int NPM = MAG ? NPART : NPART - 1;  // extrema in use (no resistive limit without B)
-    __shared__ double pmw_s[STAGE == 2 ? NPM : 1][STAGE == 2 ? TY : 1];
-
+    double ddxy_s[STAGE == 2 ? ROWS : 1][66];
+    double pmw_s[STAGE == 2 ? NPM : 1][STAGE == 2 ? TY : 1];
     // stage 3 screen thresholds from the previous reduction (DevScal); read
     // from LDS where they are used, so they hold no register across the loop
-    __shared__ double scr_s[NSC];
+    double scr_s[NSC];
+};
+
+// One tile t (0 <= t < a.nblocks) of one RK stage; the caller maps blocks to
+// tiles and provides the LDS.  ZMODE: 0 = Z' is the next stage's mix, 1 = Z
+// keeps the saved state (a step truncated after stage 1), 2 = Z is not
+// written (last stage).
+// HEAT: the ITCON=2 embedded heat loss term is added to FT; DEN: density
+// diffusion is added to FR.  Separate instantiations, so that the default
+// kernels carry none of their registers or LDS.
+template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false, bool DEN = false>
+__device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, const Metrics& M,
+                                           const int t,
+                                           StageLds<TY, STAGE, MAG, HEAT, DEN>& L) {
+    using LT = StageLds<TY, STAGE, MAG, HEAT, DEN>;
+    constexpr int ROWS = LT::ROWS;
+    constexpr int NW = LT::NW;
+    constexpr int NF = MAG ? 8 : 5;  // primaries per state
+    constexpr int SH_RO = MAG ? NSH_MAG : NSH;  // DEN only: raw RO of the plane
+    constexpr int KCMAX = LT::KCMAX;
+    constexpr int ZX_DEN = 0, ZX_HEAT = DEN ? 6 : 0;
+    constexpr int NPM = LT::NPM;
+    auto& sh = L.sh;
+    auto& red = L.red;
+    auto& zm = L.zm;
+    auto& zx = L.zx;
+    auto& hgx_s = L.hgx_s;
+    auto& hgy_s = L.hgy_s;
+    auto& ddxy_s = L.ddxy_s;
+    auto& pmw_s = L.pmw_s;
+    auto& scr_s = L.scr_s;
     // Stage 1 takes the dt reduced at the end of the previous step (or by the
     // standalone dt pass) and makes it the step's dt.
     const double dt = (STAGE == 0) ? a.scal->dt_next : a.scal->dt;
@@ -784,9 +804,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
 template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false, bool DEN = false>
 __global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, const Consts C,
                                                          const Metrics M) {
+    __shared__ StageLds<TY, STAGE, MAG, HEAT, DEN> lds;
     const int t = logical_block();
     if (t >= a.nblocks) return;
-    stage_tile<TY, STAGE, ZMODE, MAG, HEAT, DEN>(a, C, M, t);
+    stage_tile<TY, STAGE, ZMODE, MAG, HEAT, DEN>(a, C, M, t, lds);
 }
 
 }  // namespace R3D_KNS

@@ -139,4 +139,96 @@ def test_ensemble_workload_64_members_200_steps():
         compare_member(ens, m, *single_run(a, 200), "64-member workload")
         if m % 8 == 7:
             say(f"[gpu]   members 0..{m} identical, {time.perf_counter() - t0:.1f} s")
+    # the same 200 steps in one persistent launch, at the autotuned tiles
+    pers = pkg.Ensemble([pkg.Run(a) for a in members])
+    pers.set_tiles(*ens.tiles()[:2])
+    pers.set_persistent(True)
+    pers.step(200)
+    pers.sync()
+    say(f"[gpu]   persistent launch of 200 steps done, {time.perf_counter() - t0:.1f} s")
+    for m in range(len(members)):
+        want = ens.download_global(m, XZ_FIELDS)
+        compare_member(pers, m, want, ens.scalars(m), "64-member workload, persistent")
+    pers.close()
+    ens.close()
+
+
+class _MarginRun:
+    """A Run whose configuration carries a time-step screen margin."""
+
+    def __init__(self, args, margin):
+        self._run = pkg.Run(args)
+        self._margin = margin
+
+    def cfg(self):
+        c = self._run.cfg()
+        c.dt_screen = self._margin
+        return c
+
+    def __getattr__(self, name):
+        return getattr(self._run, name)
+
+
+@pytest.mark.parametrize("tiles", [(8, 0), (4, 16), (9, 32)], ids=["ty8", "ty4_kc16", "ty9_kc32"])
+def test_persistent_launch_members_match_single_runs(tiles):
+    """One cooperative launch per step() call: bit-identical to the members'
+    own runs, also when alternated with graph replay."""
+    say(f"[gpu] persistent ensemble of {len(SMALL)} uneven members, tiles {tiles}")
+    want = [single_run(a, 7) for a in SMALL]
+    ens = pkg.Ensemble([pkg.Run(a) for a in SMALL])
+    ens.set_tiles(*tiles)
+    ens.set_persistent(True)
+    ens.step(3)
+    ens.set_persistent(False)
+    ens.step(1)
+    ens.set_persistent(True)
+    ens.step(2)
+    ens.step(1)
+    for m in range(len(SMALL)):
+        compare_member(ens, m, *want[m], f"persistent, tiles {tiles}")
+    ens.close()
+
+
+def test_persistent_launch_exact_dt_pass():
+    """Members whose screen margin sends every step through the exact dt pass
+    (-0.5) next to members with the default margin, in one persistent launch:
+    the redo phase runs for the flagged members only and every member equals
+    its own run."""
+    margins = [-0.5, 0.0, -0.5, 0.0]
+    say("[gpu] persistent ensemble, exact dt pass on half the members")
+    want = [single_run(a, 5) for a in SMALL]
+    ens = pkg.Ensemble([_MarginRun(a, mg) for a, mg in zip(SMALL, margins)])
+    ens.set_persistent(True)
+    ens.step(5)
+    for m in range(len(SMALL)):
+        compare_member(ens, m, *want[m], f"persistent, margin {margins[m]}")
+    ens.close()
+
+
+def test_persistent_launch_mhd_members():
+    mag = ["--lmag", "1", "--rm", "50", "--beta", "10", "--ampb", "1", "--bfh", "2",
+           "--bzp", "10", "--lrot", "1", "--ry", "0.1", "--ang", "0"]
+    members = [
+        ["--npx", "32", "--npy", "32", "--npz", "48", "--npey", "2", "--npez", "2", "--seed",
+         "-11"] + mag,
+        ["--npx", "70", "--npy", "24", "--npz", "40", "--npey", "1", "--npez", "2", "--seed",
+         "-12", "--re", "3.5"] + mag,
+    ]
+    say("[gpu] persistent MHD ensemble")
+    xs = ["bx", "by", "bz"]
+    want, want_b = [], []
+    for a in members:
+        want.append(single_run(a, 4))
+        sim = pkg.Run(a).make_sim()
+        sim.step(4)
+        want_b.append(sim.download_global(xs))
+        sim.close()
+    ens = pkg.Ensemble([pkg.Run(a) for a in members])
+    ens.set_persistent(True)
+    ens.step(4)
+    for m in range(len(members)):
+        compare_member(ens, m, *want[m], "persistent mhd")
+        got = ens.download_global(m, xs)
+        for n in xs:
+            assert np.array_equal(got[n], want_b[m][n]), (m, n)
     ens.close()
