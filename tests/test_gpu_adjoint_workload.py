@@ -94,3 +94,37 @@ def test_parameter_gradient(workload, param, step):
     rel = abs(ad - fd) / abs(fd)
     say(f"[gpu]   dJ/d{param}: adjoint {ad:.10e} fd {fd:.10e} rel {rel:.2e}")
     assert rel <= 1e-6, (param, ad, fd)
+
+
+def test_total_theta_derivative_at_workload_size(workload):
+    """theta sets the static profile (rho, T, B and the fixed bottom
+    temperature tb) as well as the heat-flux conditions.  With the workload's
+    perturbation delta held fixed on top of the profile, the autograd total
+    dJ/dtheta (adjoint for the 200-step run with its CFL dt, central
+    difference of the host setup for the profile, dJ/dtb from the transposed
+    boundary pass) against central differences of the whole pipeline, the
+    profile rebuilt at theta +- 1e-5 theta."""
+    torch, prob, delta0, _, _, dth_bc, _ = workload
+    dev = delta0.device
+    delta = delta0.detach().cpu()
+    re = torch.tensor(prob.cfg.re, dtype=torch.float64)
+    th0 = prob.cfg.theta
+
+    def J_of(theta):
+        st = prob.static_state(theta)
+        fields = {n: st[n] for n in ("ro", "bx", "by", "bz")}
+        fields["tt"] = st["tt"] * (1.0 + delta)
+        return prob.kinetic_energy(fields, re, theta, st["tb"])
+
+    th = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+    J = J_of(th)
+    J.backward()
+    ad = float(th.grad)
+    h = 1e-5 * th0
+    with torch.no_grad():
+        fd = float(J_of(torch.tensor(th0 + h, dtype=torch.float64)) -
+                   J_of(torch.tensor(th0 - h, dtype=torch.float64))) / (2 * h)
+    rel = abs(ad - fd) / abs(fd)
+    say(f"[gpu]   total dJ/dtheta: adjoint {ad:.10e} fd {fd:.10e} rel {rel:.2e} "
+        f"(boundary part alone {dth_bc:.10e}); J {float(J.detach()):.10e} on {dev}")
+    assert rel <= 1e-6, (ad, fd)
