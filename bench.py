@@ -289,8 +289,9 @@ def measure_ensemble(pkg, nmem, steps, warmup):
     log(f"[bench] ensemble of {len(runs)} members, {cells} cells")
     ens = pkg.Ensemble(runs)
     ms_tuned = ens.autotune(3)
-    ty, kc, nt = ens.tiles()
-    log(f"[bench] autotuned tiles: ty={ty} kc={kc} ({nt} tiles/stage), {ms_tuned:.3f} ms/step")
+    ty, kc, tw, nt = ens.tile_shape()
+    log(f"[bench] autotuned tiles: ty={ty} kc={kc} tw={tw} ({nt} tiles/stage), "
+        f"{ms_tuned:.3f} ms/step")
     ens.step(warmup)
     ens.sync()
     t0 = time.perf_counter()
@@ -320,7 +321,7 @@ def measure_ensemble(pkg, nmem, steps, warmup):
         seq += time.perf_counter() - t0
         sim.close()
         r.close()
-    return {"members": len(runs), "cells": cells, "wall": wall, "seq": seq, "tiles": (ty, kc),
+    return {"members": len(runs), "cells": cells, "wall": wall, "seq": seq, "tiles": (ty, kc, tw),
             "mem": mem, "wall_persistent": wall_p}
 
 
@@ -329,7 +330,7 @@ def run_ensemble(a):
     replay) next to the same members stepped one after another."""
     pkg = load_package()
     e = measure_ensemble(pkg, a.ensemble, a.steps, a.warmup)
-    cells, wall, seq, (ty, kc), mem = e["cells"], e["wall"], e["seq"], e["tiles"], e["mem"]
+    cells, wall, seq, (ty, kc, tw), mem = e["cells"], e["wall"], e["seq"], e["tiles"], e["mem"]
     value = cells * 3 * a.steps / wall
     seq_value = cells * 3 * a.steps / seq
     achieved = BYTES_PER_CELL_STAGE * cells * 3 * a.steps / wall / 1e9
@@ -340,7 +341,7 @@ def run_ensemble(a):
         "data": "synthetic: seeded per-member polytropes and ran2 perturbations",
         "config": {"workload": f"ensemble of {e['members']} hydro members (48..96)^2 x 96..192, "
                                "Re in [1,4], theta in [0.1,0.3], virtual 1x2 each",
-                   "global_cells": cells, "tiles": [ty, kc], "graph": True,
+                   "global_cells": cells, "tiles": [ty, kc, tw], "graph": True,
                    "device_bytes": mem, "sequential_value": seq_value,
                    "speedup_vs_sequential": value / seq_value,
                    "persistent_value": cells * 3 * a.steps / e["wall_persistent"],

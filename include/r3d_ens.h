@@ -41,10 +41,19 @@ int r3d_ens_set_scalars(r3d_ens_ctx* e, int member, const r3d_hip_scalars* s);
 int r3d_ens_get_scalars(r3d_ens_ctx* e, int member, r3d_hip_scalars* s);
 
 /* Tile shape: ty rows per tile (4, 8 or 9 for hydro, 5 for MHD) and kc planes
- * per z chunk (8..128; 0 = per-member default).  ntiles = tiles per stage
- * launch over all members. */
+ * per z chunk (8..128; 0 = per-member default), 64 columns wide.  ntiles =
+ * tiles per stage launch over all members. */
 int r3d_ens_set_tiles(r3d_ens_ctx* e, int ty, int kc);
 int r3d_ens_get_tiles(r3d_ens_ctx* e, int* ty, int* kc, int* ntiles);
+
+/* The same with the tile width tw: 64 (the above), or, for hydro ensembles,
+ * 32 columns with two rows per wave (ty 8 or 16) or 16 columns with four
+ * (ty 16 or 28), which leave fewer lanes idle on members whose width is not
+ * a multiple of 64 (80 and 96 wide: 83% and 100% of the lanes busy with 32
+ * columns, 100% with 16, against 62.5% and 75%).  Results do not depend on
+ * the tile shape. */
+int r3d_ens_set_tile_shape(r3d_ens_ctx* e, int ty, int kc, int tw);
+int r3d_ens_get_tile_shape(r3d_ens_ctx* e, int* ty, int* kc, int* tw, int* ntiles);
 
 /* Time every candidate tile shape over trial_steps full steps (after one
  * untimed step), keep the fastest, and restore every member's state and

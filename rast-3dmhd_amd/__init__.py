@@ -114,6 +114,9 @@ def load():
         "r3d_ens_get_scalars": [vp, ip, ctypes.POINTER(Scalars)],
         "r3d_ens_set_tiles": [vp, ip, ip],
         "r3d_ens_get_tiles": [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip)],
+        "r3d_ens_set_tile_shape": [vp, ip, ip, ip],
+        "r3d_ens_get_tile_shape": [vp, ctypes.POINTER(ip), ctypes.POINTER(ip),
+                                   ctypes.POINTER(ip), ctypes.POINTER(ip)],
         "r3d_ens_autotune": [vp, ip, ctypes.POINTER(ctypes.c_double)],
         "r3d_ens_set_graph": [vp, ip],
         "r3d_ens_set_persistent": [vp, ip],
@@ -344,8 +347,16 @@ class Ensemble:
     def sync(self):
         _check(load().r3d_ens_sync(self.h))
 
-    def set_tiles(self, ty, kc=0):
-        _check(load().r3d_ens_set_tiles(self.h, ty, kc))
+    def set_tiles(self, ty, kc=0, tw=64):
+        """ty rows, kc planes (0: default), tw columns (64, or 32 with two
+        rows per wave)."""
+        _check(load().r3d_ens_set_tile_shape(self.h, ty, kc, tw))
+
+    def tile_shape(self):
+        """(rows, planes, columns, tiles per stage) of the current tiling."""
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(load().r3d_ens_get_tile_shape(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def tiles(self):
         ty, kc, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -61,10 +61,10 @@ __device__ __forceinline__ StageArgs member_args(const EnsMember& e, int cur, in
 // One stage of a full step for every member.  Tiles of one member are
 // contiguous in the global order, and the XCD-aware block order keeps each
 // member's tiles together on one XCD where it can.
-template <int TY, int STAGE, bool MAG>
-__global__ __launch_bounds__((TY + 3) * 64) void k_stage_ens(const EnsMember* __restrict__ mem,
-                                                             int nmem, int ntiles, int cur) {
-    __shared__ StageLds<TY, STAGE, MAG, false, false> lds;
+template <int TY, int STAGE, bool MAG, int TW>
+__global__ __launch_bounds__((TileShape<TY, TW>::NW * 64)) void k_stage_ens(
+    const EnsMember* __restrict__ mem, int nmem, int ntiles, int cur) {
+    __shared__ StageLds<TY, STAGE, MAG, false, false, TW> lds;
     const int T = logical_block();
     if (T >= ntiles) return;
     int lo = 0, hi = nmem - 1;  // last member whose first tile is <= T
@@ -76,7 +76,7 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_stage_ens(const EnsMember* __
     const EnsMember& e = mem[lo];
     constexpr int ZMODE = STAGE < 2 ? 0 : 2;
     const StageArgs a = member_args(e, cur, STAGE, ZMODE, STAGE == 2);
-    stage_tile<TY, STAGE, ZMODE, MAG>(a, e.C, e.M, T - e.tile0, lds);
+    stage_tile<TY, STAGE, ZMODE, MAG, false, false, TW>(a, e.C, e.M, T - e.tile0, lds);
 }
 
 // Last member m with key(mem[m]) <= v (keys ascending from mem[0] = 0).
@@ -112,15 +112,15 @@ __device__ __forceinline__ int find_member(const EnsMember* mem, int nmem, int v
 // loops contain none.  Results are those of the per-stage launches bit for
 // bit (the same tile and cell code; the min / max reductions do not depend on
 // how the partials are grouped).  work[0..8] must be zero at launch.
-template <int TY, bool MAG>
-__global__ __launch_bounds__((TY + 3) * 64) void k_ens_run(const EnsMember* __restrict__ mem,
-                                                           int nmem, int ntiles, int nbunits,
-                                                           int cur, int nsteps, int* work) {
-    constexpr int NT = (TY + 3) * 64;
+template <int TY, bool MAG, int TW>
+__global__ __launch_bounds__((TileShape<TY, TW>::NW * 64)) void k_ens_run(
+    const EnsMember* __restrict__ mem, int nmem, int ntiles, int nbunits, int cur, int nsteps,
+    int* work) {
+    constexpr int NT = TileShape<TY, TW>::NW * 64;
     __shared__ union {
-        StageLds<TY, 0, MAG, false, false> s0;
-        StageLds<TY, 1, MAG, false, false> s1;
-        StageLds<TY, 2, MAG, false, false> s2;
+        StageLds<TY, 0, MAG, false, false, TW> s0;
+        StageLds<TY, 1, MAG, false, false, TW> s1;
+        StageLds<TY, 2, MAG, false, false, TW> s2;
     } lds;
     __shared__ int grab;
     cooperative_groups::grid_group grid = cooperative_groups::this_grid();
@@ -143,9 +143,12 @@ __global__ __launch_bounds__((TY + 3) * 64) void k_ens_run(const EnsMember* __re
                 const EnsMember& e =
                     mem[find_member(mem, nmem, T, [](const EnsMember& x) { return x.tile0; })];
                 const StageArgs a = member_args(e, cur, s, s < 2 ? 0 : 2, s == 2);
-                if (s == 0) stage_tile<TY, 0, 0, MAG>(a, e.C, e.M, T - e.tile0, lds.s0);
-                else if (s == 1) stage_tile<TY, 1, 0, MAG>(a, e.C, e.M, T - e.tile0, lds.s1);
-                else stage_tile<TY, 2, 2, MAG>(a, e.C, e.M, T - e.tile0, lds.s2);
+                if (s == 0)
+                    stage_tile<TY, 0, 0, MAG, false, false, TW>(a, e.C, e.M, T - e.tile0, lds.s0);
+                else if (s == 1)
+                    stage_tile<TY, 1, 0, MAG, false, false, TW>(a, e.C, e.M, T - e.tile0, lds.s1);
+                else
+                    stage_tile<TY, 2, 2, MAG, false, false, TW>(a, e.C, e.M, T - e.tile0, lds.s2);
                 __syncthreads();  // the next tile reuses the LDS
             }
             base += n_x + per_xcd;
@@ -234,8 +237,11 @@ __global__ __launch_bounds__(1024) void k_finish_redo_ens(const EnsMember* __res
     finish_block(e.scal, e.partial, e.npart_dt, e.C, 1, 0, 0);
 }
 
+// Stage tiles of the ensemble: ty rows, kc planes (0: 32), tw columns (64,
+// or 32 with two rows per wave for members whose width leaves much of a
+// 64-column tile idle).
 struct Tiling {
-    int ty, kc;
+    int ty, kc, tw;
 };
 
 }  // namespace
@@ -248,7 +254,7 @@ struct r3d_ens_ctx {
     int* d_work = nullptr;  // persistent launch: tile counters per XCD, exact-pass flag
     hipStream_t stream = nullptr;
     int device = 0, lmag = 0;
-    Tiling tile{8, 0};
+    Tiling tile{8, 0, 64};
     int ntiles = 0, grid = 0, nbb_max = 0;
     int cur = 0;
     bool use_graph = true;
@@ -274,7 +280,8 @@ void drop_graphs(r3d_ens_ctx* e) {
 }
 
 // Threads of a persistent-launch block at the current tiling.
-int persistent_threads(const r3d_ens_ctx* e) { return (e->tile.ty + 3) * 64; }
+int wave_count(int ty, int tw) { return tw == 64 ? ty + 3 : ty / (64 / tw) + 2; }
+int persistent_threads(const r3d_ens_ctx* e) { return wave_count(e->tile.ty, e->tile.tw) * 64; }
 
 // Rebuild the member table for the current tiling and parity.
 int build_table(r3d_ens_ctx* e) {
@@ -290,7 +297,7 @@ int build_table(r3d_ens_ctx* e) {
         for (int f = 0; f < NXF; ++f) d.z.q[f] = c->z[f];
         d.partial = e->part[m];
         d.scal = c->scal;
-        d.ntx = (c->cfg.npx + 63) / 64;
+        d.ntx = (c->cfg.npx + e->tile.tw - 1) / e->tile.tw;
         d.nty = (c->cfg.npy + e->tile.ty - 1) / e->tile.ty;
         d.kc = member_kc(e, c);
         d.nkc = (c->cfg.npz + d.kc - 1) / d.kc;
@@ -316,19 +323,19 @@ int build_table(r3d_ens_ctx* e) {
     return R3D_OK;
 }
 
-template <int TY, bool MAG>
+template <int TY, bool MAG, int TW = 64>
 void launch_stages_ty(r3d_ens_ctx* e, int stage, int cur) {
-    const dim3 blk((TY + 3) * 64);
+    const dim3 blk(TileShape<TY, TW>::NW * 64);
     const int n = (int)e->mem.size();
     if (stage == 0)
-        hipLaunchKernelGGL((k_stage_ens<TY, 0, MAG>), dim3(e->grid), blk, 0, e->stream, e->d_desc,
-                           n, e->ntiles, cur);
+        hipLaunchKernelGGL((k_stage_ens<TY, 0, MAG, TW>), dim3(e->grid), blk, 0, e->stream,
+                           e->d_desc, n, e->ntiles, cur);
     else if (stage == 1)
-        hipLaunchKernelGGL((k_stage_ens<TY, 1, MAG>), dim3(e->grid), blk, 0, e->stream, e->d_desc,
-                           n, e->ntiles, cur);
+        hipLaunchKernelGGL((k_stage_ens<TY, 1, MAG, TW>), dim3(e->grid), blk, 0, e->stream,
+                           e->d_desc, n, e->ntiles, cur);
     else
-        hipLaunchKernelGGL((k_stage_ens<TY, 2, MAG>), dim3(e->grid), blk, 0, e->stream, e->d_desc,
-                           n, e->ntiles, cur);
+        hipLaunchKernelGGL((k_stage_ens<TY, 2, MAG, TW>), dim3(e->grid), blk, 0, e->stream,
+                           e->d_desc, n, e->ntiles, cur);
 }
 
 // The launches of one full step starting from parity cur.
@@ -336,6 +343,10 @@ int enqueue_step(r3d_ens_ctx* e, int cur) {
     const int n = (int)e->mem.size();
     for (int s = 0; s < 3; ++s) {
         if (e->lmag) launch_stages_ty<TILE_Y_MAG, true>(e, s, cur);
+        else if (e->tile.tw == 32 && e->tile.ty == 8) launch_stages_ty<8, false, 32>(e, s, cur);
+        else if (e->tile.tw == 32) launch_stages_ty<16, false, 32>(e, s, cur);
+        else if (e->tile.tw == 16 && e->tile.ty == 16) launch_stages_ty<16, false, 16>(e, s, cur);
+        else if (e->tile.tw == 16) launch_stages_ty<28, false, 16>(e, s, cur);
         else if (e->tile.ty == 4) launch_stages_ty<4, false>(e, s, cur);
         else if (e->tile.ty == 8) launch_stages_ty<8, false>(e, s, cur);
         else launch_stages_ty<9, false>(e, s, cur);
@@ -358,10 +369,10 @@ int enqueue_step(r3d_ens_ctx* e, int cur) {
 // nsteps full steps in one cooperative launch (k_ens_run): as many blocks as
 // can be resident at once (a multiple of 8, at most one per tile), each
 // looping over its share of every phase.
-template <int TY, bool MAG>
+template <int TY, bool MAG, int TW = 64>
 int launch_persistent_ty(r3d_ens_ctx* e, int nsteps) {
-    constexpr int NT = (TY + 3) * 64;
-    const void* fn = reinterpret_cast<const void*>(&k_ens_run<TY, MAG>);
+    constexpr int NT = TileShape<TY, TW>::NW * 64;
+    const void* fn = reinterpret_cast<const void*>(&k_ens_run<TY, MAG, TW>);
     int coop = 0, ncu = 0, per_cu = 0;
     HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, e->device));
     if (!coop) return set_err(R3D_EUNSUP, "device %d has no cooperative launch", e->device);
@@ -382,6 +393,10 @@ int launch_persistent_ty(r3d_ens_ctx* e, int nsteps) {
 
 int launch_persistent(r3d_ens_ctx* e, int nsteps) {
     if (e->lmag) return launch_persistent_ty<TILE_Y_MAG, true>(e, nsteps);
+    if (e->tile.tw == 32 && e->tile.ty == 8) return launch_persistent_ty<8, false, 32>(e, nsteps);
+    if (e->tile.tw == 32) return launch_persistent_ty<16, false, 32>(e, nsteps);
+    if (e->tile.tw == 16 && e->tile.ty == 16) return launch_persistent_ty<16, false, 16>(e, nsteps);
+    if (e->tile.tw == 16) return launch_persistent_ty<28, false, 16>(e, nsteps);
     if (e->tile.ty == 4) return launch_persistent_ty<4, false>(e, nsteps);
     if (e->tile.ty == 8) return launch_persistent_ty<8, false>(e, nsteps);
     return launch_persistent_ty<9, false>(e, nsteps);
@@ -457,7 +472,7 @@ int r3d_ens_create(int nmembers, const r3d_hip_cfg* cfgs, r3d_ens_ctx** out) {
     auto* e = new r3d_ens_ctx();
     e->device = cfgs[0].device;
     e->lmag = cfgs[0].lmag;
-    e->tile = {e->lmag ? TILE_Y_MAG : TILE_Y, 0};
+    e->tile = {e->lmag ? TILE_Y_MAG : TILE_Y, 0, 64};
     auto fail = [&](int code) {
         r3d_ens_destroy(e);
         return code;
@@ -472,9 +487,13 @@ int r3d_ens_create(int nmembers, const r3d_hip_cfg* cfgs, r3d_ens_ctx** out) {
             return set_err(rc, "%s", msg.c_str());
         }
         e->mem.push_back(c);
-        // partial slots for the finest tiling (4 rows, 8 planes) or the dt pass
-        const long worst = (long)((cfgs[m].npx + 63) / 64) * ((cfgs[m].npy + 3) / 4) *
-                               ((cfgs[m].npz + 7) / 8) + 2L * c->nbb;
+        // partial slots for the finest tiling (4 rows x 64 columns, 8 x 32 or
+        // 16 x 16, 8 planes) or the dt pass
+        const long cols = std::max<long>(
+            std::max<long>((long)((cfgs[m].npx + 63) / 64) * ((cfgs[m].npy + 3) / 4),
+                           (long)((cfgs[m].npx + 31) / 32) * ((cfgs[m].npy + 7) / 8)),
+            (long)((cfgs[m].npx + 15) / 16) * ((cfgs[m].npy + 15) / 16));
+        const long worst = cols * ((cfgs[m].npz + 7) / 8) + 2L * c->nbb;
         double* p = nullptr;
         const size_t np = (size_t)std::max<long>(worst, ENS_DT_BLOCKS) * NPART;
         if (hipMalloc((void**)&p, np * sizeof(double)) != hipSuccess)
@@ -554,17 +573,27 @@ int r3d_ens_get_scalars(r3d_ens_ctx* e, int m, r3d_hip_scalars* s) {
     return r3d_hip_get_scalars(e->mem[m], s);
 }
 
-int r3d_ens_set_tiles(r3d_ens_ctx* e, int ty, int kc) {
+int r3d_ens_set_tile_shape(r3d_ens_ctx* e, int ty, int kc, int tw) {
     if (!e) return set_err(R3D_EINVAL, "null ensemble");
-    const bool ok_ty = e->lmag ? ty == TILE_Y_MAG : (ty == 4 || ty == 8 || ty == 9);
-    if (!ok_ty) return set_err(R3D_EINVAL, "tile rows %d not built for this ensemble", ty);
+    bool ok;
+    if (e->lmag) ok = tw == 64 && ty == TILE_Y_MAG;
+    else if (tw == 64) ok = ty == 4 || ty == 8 || ty == 9;
+    else if (tw == 32) ok = ty == 8 || ty == 16;
+    else ok = tw == 16 && (ty == 16 || ty == 28);
+    if (!ok)
+        return set_err(R3D_EINVAL, "tile of %d rows x %d columns not built for this ensemble",
+                       ty, tw);
     if (kc != 0 && (kc < 8 || kc > 128)) return set_err(R3D_EINVAL, "kc %d outside 8..128", kc);
-    e->tile = {ty, kc};
+    e->tile = {ty, kc, tw};
     e->ntiles = 0;
     return R3D_OK;
 }
 
-int r3d_ens_get_tiles(r3d_ens_ctx* e, int* ty, int* kc, int* ntiles) {
+int r3d_ens_set_tiles(r3d_ens_ctx* e, int ty, int kc) {
+    return r3d_ens_set_tile_shape(e, ty, kc, 64);
+}
+
+int r3d_ens_get_tile_shape(r3d_ens_ctx* e, int* ty, int* kc, int* tw, int* ntiles) {
     if (!e) return set_err(R3D_EINVAL, "null ensemble");
     if (!e->ntiles) {
         const int rc = build_table(e);
@@ -572,8 +601,13 @@ int r3d_ens_get_tiles(r3d_ens_ctx* e, int* ty, int* kc, int* ntiles) {
     }
     if (ty) *ty = e->tile.ty;
     if (kc) *kc = e->tile.kc;
+    if (tw) *tw = e->tile.tw;
     if (ntiles) *ntiles = e->ntiles;
     return R3D_OK;
+}
+
+int r3d_ens_get_tiles(r3d_ens_ctx* e, int* ty, int* kc, int* ntiles) {
+    return r3d_ens_get_tile_shape(e, ty, kc, nullptr, ntiles);
 }
 
 int r3d_ens_set_graph(r3d_ens_ctx* e, int enable) {
@@ -676,10 +710,14 @@ int r3d_ens_autotune(r3d_ens_ctx* e, int trial_steps, double* best_ms) {
     };
     std::vector<Tiling> cand;
     if (e->lmag) {
-        for (int kc : {8, 16, 32, 64}) cand.push_back({TILE_Y_MAG, kc});
+        for (int kc : {8, 16, 32, 64}) cand.push_back({TILE_Y_MAG, kc, 64});
     } else {
         for (int ty : {4, 8, 9})
-            for (int kc : {8, 16, 32, 64}) cand.push_back({ty, kc});
+            for (int kc : {8, 16, 32, 64}) cand.push_back({ty, kc, 64});
+        for (int ty : {8, 16})
+            for (int kc : {16, 32, 64}) cand.push_back({ty, kc, 32});
+        for (int ty : {16, 28})
+            for (int kc : {16, 32, 64}) cand.push_back({ty, kc, 16});
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;
     HIPCHK(hipEventCreate(&t0));

@@ -177,15 +177,30 @@ __device__ __forceinline__ void block_partials(double pm[NPART], double (*red)[N
     }
 }
 
+// Waves of a tile TW columns wide and TY rows high.  TW = 64: one row per
+// wave (waves 1..TY the flux rows, 0 and TY+1 the y-halo rows, TY+2 the
+// x-halo columns).  TW = 32 or 16: RPW = 64 / TW rows per wave (waves
+// 0..TY/RPW-1 the flux rows, TY/RPW both y-halo rows, TY/RPW+1 the x-halo
+// columns).
+template <int TY, int TW>
+struct TileShape {
+    static constexpr int RPW = 64 / TW;  // rows per wave
+    static_assert((TW == 64 || TW == 32 || TW == 16) && TY % RPW == 0 && TY + 2 <= 32,
+                  "tile width 64, 32 or 16 with whole waves of flux rows");
+    static constexpr int ROWS = TY + 2;
+    static constexpr int NFW = TY / RPW;  // flux waves
+    static constexpr int NW = TW == 64 ? TY + 3 : NFW + 2;
+};
+
 // LDS of one stage tile.  The kernel that runs the tile declares it, so that
 // a persistent kernel can run the three stages in one allocation (a union).
-template <int TY, int STAGE, bool MAG, bool HEAT, bool DEN>
+template <int TY, int STAGE, bool MAG, bool HEAT, bool DEN, int TW = 64>
 struct StageLds {
-    static constexpr int ROWS = TY + 2, NW = TY + 3;
+    static constexpr int ROWS = TY + 2, NW = TileShape<TY, TW>::NW, NFW = TileShape<TY, TW>::NFW;
     static constexpr int KCMAX = 128;
     static constexpr int NZX = (DEN ? 6 : 0) + (HEAT ? 1 : 0);
     static constexpr int NPM = MAG ? NPART : NPART - 1;  // extrema in use (no resistive limit without B)
-    double sh[2][(MAG ? NSH_MAG : NSH) + (DEN ? 1 : 0)][ROWS][66];
+    double sh[2][(MAG ? NSH_MAG : NSH) + (DEN ? 1 : 0)][ROWS][TW + 2];
     double red[NW][NPART];
     // Per-plane z metrics of this block's chunk, staged once: they are
     // uniform per plane, and fetching them inside the z loop would put a
@@ -200,16 +215,16 @@ struct StageLds {
     // registers across the z loop
     // and with LREM TTM at k-1..k+1 (row ZX_DEN + 4, offset by one plane) and HRAD
     double zx[NZX > 0 ? NZX : 1][KCMAX + 2];
-    double hgx_s[HEAT ? 66 : 1], hgy_s[HEAT ? ROWS : 1];
+    double hgx_s[HEAT ? TW + 2 : 1], hgy_s[HEAT ? ROWS : 1];
     // Stage 3: the column's min(ddx, ddy) and each flux row's running dt
     // extrema live in LDS rather than in registers across the z loop.  Held
     // in registers they are spilled, and a scratch reload after a plane's
     // stores waits for all of them (vector memory operations complete in
-    // issue order); an LDS access does not.  The extrema are kept per row
-    // (wave), merged by a wave reduction on the rare planes where some cell
-    // of the row passes the time-step screen.
-    double ddxy_s[STAGE == 2 ? ROWS : 1][66];
-    double pmw_s[STAGE == 2 ? NPM : 1][STAGE == 2 ? TY : 1];
+    // issue order); an LDS access does not.  The extrema are kept per flux
+    // wave, merged by a wave reduction on the rare planes where some cell
+    // of the wave passes the time-step screen.
+    double ddxy_s[STAGE == 2 ? ROWS : 1][TW + 2];
+    double pmw_s[STAGE == 2 ? NPM : 1][STAGE == 2 ? NFW : 1];
     // stage 3 screen thresholds from the previous reduction (DevScal); read
     // from LDS where they are used, so they hold no register across the loop
     double scr_s[NSC];
@@ -222,11 +237,11 @@ struct StageLds {
 // HEAT: the ITCON=2 embedded heat loss term is added to FT; DEN: density
 // diffusion is added to FR.  Separate instantiations, so that the default
 // kernels carry none of their registers or LDS.
-template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false, bool DEN = false>
+template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false, bool DEN = false, int TW = 64>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, const Metrics& M,
                                            const int t,
-                                           StageLds<TY, STAGE, MAG, HEAT, DEN>& L) {
-    using LT = StageLds<TY, STAGE, MAG, HEAT, DEN>;
+                                           StageLds<TY, STAGE, MAG, HEAT, DEN, TW>& L) {
+    using LT = StageLds<TY, STAGE, MAG, HEAT, DEN, TW>;
     constexpr int ROWS = LT::ROWS;
     constexpr int NW = LT::NW;
     constexpr int NF = MAG ? 8 : 5;  // primaries per state
@@ -257,29 +272,41 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     const int ty = rem % a.nty;
     const int kz = rem / a.nty;
     const Geo g = a.g;
-    const int i0 = 1 + 64 * tx, j0 = 1 + TY * ty;
+    const int i0 = 1 + TW * tx, j0 = 1 + TY * ty;
     const int k0 = 1 + a.kc * kz;
     const int k1 = min(g.npz, k0 + a.kc - 1);
 
     // the wave index is uniform over the wave; from a scalar register, the
     // role branch below is a scalar branch
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
+    constexpr int NFW = LT::NFW;
+    // LDS row and column of this lane (column 1..TW inside the tile)
     int r, c, gi, gj;
     bool act;
-    if (wv < ROWS) {
-        r = wv;
-        c = ln + 1;
-        gi = i0 + ln;
+    if (TW == 64 ? wv < ROWS : wv <= NFW) {
+        if constexpr (TW == 64) {
+            r = wv;
+            c = ln + 1;
+        } else {  // RPW rows per wave; wave NFW holds the two y-halo rows
+            constexpr int RPW = TileShape<TY, TW>::RPW;
+            const int q = ln / TW;
+            r = wv < NFW ? 1 + RPW * wv + q : (q == 1 ? TY + 1 : 0);
+            c = ln % TW + 1;
+            act = wv < NFW || q < 2;  // y-halo wave: lanes past the two rows idle
+        }
+        gi = i0 + c - 1;
         gj = j0 - 1 + r;
-        act = true;
+        if constexpr (TW == 64) act = true;
     } else {
         act = ln < 2 * ROWS;
         r = act ? (ln >> 1) : 0;
-        c = (ln & 1) ? 65 : 0;
-        gi = (ln & 1) ? i0 + 64 : i0 - 1;
+        c = (ln & 1) ? TW + 1 : 0;
+        gi = (ln & 1) ? i0 + TW : i0 - 1;
         gj = j0 - 1 + r;
     }
-    const bool heavy = (wv >= 1) && (wv <= TY);
+    const bool heavy = TW == 64 ? (wv >= 1) && (wv <= TY) : wv < NFW;
+    // the flux wave's slot for its running dt extrema
+    const int fw = TW == 64 ? wv - 1 : wv;
     const bool outp = heavy && gi <= g.npx && gj <= g.npy;
     // the column this lane reads: periodic wrap at the x/y edges; lanes past
     // the wrap (partial edge tiles) read column/row 1 and their stores go to
@@ -321,13 +348,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     if constexpr (HEAT) {
         for (int q = threadIdx.x; q < KCMAX; q += blockDim.x)
             if (k0 + q <= k1) zx[ZX_HEAT][q] = M.hgz[k0 + q];
-        for (int q = threadIdx.x; q < 66 + ROWS; q += blockDim.x) {
-            if (q < 66) {  // column q: global i0 - 1 + q, wrapped as the lanes read it
+        for (int q = threadIdx.x; q < TW + 2 + ROWS; q += blockDim.x) {
+            if (q < TW + 2) {  // column q: global i0 - 1 + q, wrapped as the lanes read it
                 const int gi2 = i0 - 1 + q;
                 hgx_s[q] = M.hgx[gi2 == 0 ? g.npx : (gi2 > g.npx ? 1 : gi2)];
             } else {
-                const int gj2 = j0 - 1 + (q - 66);
-                hgy_s[q - 66] = M.hgy[gj2 == 0 ? g.npy : (gj2 > g.npy ? 1 : gj2)];
+                const int gj2 = j0 - 1 + (q - TW - 2);
+                hgy_s[q - TW - 2] = M.hgy[gj2 == 0 ? g.npy : (gj2 > g.npy ? 1 : gj2)];
             }
         }
     }
@@ -342,8 +369,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     if (STAGE == 2 && heavy) {
         ddxy_s[r][c] = dmin(M.ddx[li], M.ddy[lj]);
         if (ln == 0) {
-            pmw_s[0][r - 1] = 0.0;
-            for (int q = 1; q < NPM; ++q) pmw_s[q][r - 1] = 1e300;
+            pmw_s[0][fw] = 0.0;
+            for (int q = 1; q < NPM; ++q) pmw_s[q][fw] = 1e300;
         }
     }
 
@@ -363,10 +390,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
     const int r_all = r, c_all = c;
     auto zloop = [&](auto heavy_c) {
     constexpr bool HEAVY = decltype(heavy_c)::value;
-    // a flux row's LDS row is its (uniform) wave index and its column the
-    // lane + 1: no per-lane row index has to stay live in that loop
-    const int r = HEAVY ? wv : r_all;
-    const int c = HEAVY ? ln + 1 : c_all;
+    // with 64-column tiles a flux row's LDS row is its (uniform) wave index
+    // and its column the lane + 1: no per-lane row index has to stay live in
+    // that loop
+    const int r = HEAVY && TW == 64 ? wv : r_all;
+    const int c = HEAVY && TW == 64 ? ln + 1 : c_all;
     // Every load issued before the loop has landed before it starts.  The
     // compiler's wait counts at the loop head merge the entry path with the
     // back edge: a load still in flight on entry would otherwise make every
@@ -774,9 +802,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                     double vm[NPM];
                     for (int v = 1; v < NPM; ++v) vm[v] = wave_min(q[v]);
                     if (ln == 0) {
-                        pmw_s[0][r - 1] = dmax(pmw_s[0][r - 1], v0);
+                        pmw_s[0][fw] = dmax(pmw_s[0][fw], v0);
                         for (int v = 1; v < NPM; ++v)
-                            pmw_s[v][r - 1] = dmin(pmw_s[v][r - 1], vm[v]);
+                            pmw_s[v][fw] = dmin(pmw_s[v][fw], vm[v]);
                     }
                 }
             }
@@ -795,7 +823,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
 
     if (STAGE == 2 && a.want_dt) {
         if (heavy)
-            for (int q = 0; q < NPM; ++q) pm[q] = pmw_s[q][r - 1];
+            for (int q = 0; q < NPM; ++q) pm[q] = pmw_s[q][fw];
         block_partials<NW>(pm, red, a.partial, t);
     }
 }

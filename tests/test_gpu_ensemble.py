@@ -59,8 +59,11 @@ SMALL = [
 ]
 
 
-@pytest.mark.parametrize("tiles", [(9, 0), (4, 16), (8, 64), (4, 8), (9, 32)],
-                         ids=["ty9_default", "ty4_kc16", "ty8_kc64", "ty4_kc8", "ty9_kc32"])
+@pytest.mark.parametrize("tiles", [(9, 0), (4, 16), (8, 64), (4, 8), (9, 32), (16, 32, 32),
+                                   (8, 16, 32), (16, 0, 32), (28, 32, 16), (16, 16, 16)],
+                         ids=["ty9_default", "ty4_kc16", "ty8_kc64", "ty4_kc8", "ty9_kc32",
+                              "tw32_ty16_kc32", "tw32_ty8_kc16", "tw32_ty16_default",
+                              "tw16_ty28_kc32", "tw16_ty16_kc16"])
 def test_ensemble_members_match_single_runs(tiles):
     say(f"[gpu] ensemble of {len(SMALL)} uneven members, tiles {tiles}")
     want = [single_run(a, 6) for a in SMALL]
@@ -82,9 +85,10 @@ def test_autotune_restores_state_and_keeps_results():
     ens = pkg.Ensemble([pkg.Run(a) for a in SMALL])
     ens.step(1)
     ms = ens.autotune(2)
-    ty, kc, nt = ens.tiles()
-    say(f"[gpu]   chose ty={ty} kc={kc} ({nt} tiles), {ms:.3f} ms/step")
-    assert ty in (4, 8, 9) and kc in (8, 16, 32, 64) and ms > 0
+    ty, kc, tw, nt = ens.tile_shape()
+    say(f"[gpu]   chose ty={ty} kc={kc} tw={tw} ({nt} tiles), {ms:.3f} ms/step")
+    assert (tw, ty) in ((64, 4), (64, 8), (64, 9), (32, 8), (32, 16), (16, 16), (16, 28))
+    assert kc in (8, 16, 32, 64) and ms > 0
     ens.step(3)
     for m in range(len(SMALL)):
         compare_member(ens, m, *want[m], "after autotune")
@@ -130,7 +134,7 @@ def test_ensemble_workload_64_members_200_steps():
     ens = pkg.Ensemble([pkg.Run(a) for a in members])
     say(f"[gpu]   set up in {time.perf_counter() - t0:.1f} s")
     ms = ens.autotune(2)
-    say(f"[gpu]   autotuned {ens.tiles()} {ms:.3f} ms/step, {time.perf_counter() - t0:.1f} s")
+    say(f"[gpu]   autotuned {ens.tile_shape()} {ms:.3f} ms/step, {time.perf_counter() - t0:.1f} s")
     for done in range(0, 200, 50):
         ens.step(50)
         ens.sync()
@@ -141,7 +145,7 @@ def test_ensemble_workload_64_members_200_steps():
             say(f"[gpu]   members 0..{m} identical, {time.perf_counter() - t0:.1f} s")
     # the same 200 steps in one persistent launch, at the autotuned tiles
     pers = pkg.Ensemble([pkg.Run(a) for a in members])
-    pers.set_tiles(*ens.tiles()[:2])
+    pers.set_tiles(*ens.tile_shape()[:3])
     pers.set_persistent(True)
     pers.step(200)
     pers.sync()
@@ -169,7 +173,9 @@ class _MarginRun:
         return getattr(self._run, name)
 
 
-@pytest.mark.parametrize("tiles", [(8, 0), (4, 16), (9, 32)], ids=["ty8", "ty4_kc16", "ty9_kc32"])
+@pytest.mark.parametrize("tiles", [(8, 0), (4, 16), (9, 32), (16, 32, 32), (8, 0, 32),
+                                   (28, 32, 16)],
+                         ids=["ty8", "ty4_kc16", "ty9_kc32", "tw32_ty16", "tw32_ty8", "tw16_ty28"])
 def test_persistent_launch_members_match_single_runs(tiles):
     """One cooperative launch per step() call: bit-identical to the members'
     own runs, also when alternated with graph replay."""
