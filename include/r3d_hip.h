@@ -73,7 +73,9 @@ typedef struct r3d_hip_cfg {
                                    npez >= 2 and toff > 0 only */
     /* Stage-kernel tiling (results do not depend on it; 0 = chosen by the
      * library).  tile_rows: y rows per workgroup tile, 8 or 9 for hydro (8
-     * with itcon 2, id or rlax), 5 with lmag.  zchunk: z planes per
+     * with itcon 2, id or rlax; 64 columns), 12 with lmag (32 columns, two
+     * rows per wave) or 5 with lmag (64 columns; not with itcon 2, id, rlax or
+     * variant 1).  zchunk: z planes per
      * workgroup, 8, 16, 32, 64 or 128.  No reference counterpart: the
      * reference GPU launches one thread per cell (src/gpu.cpp). */
     int tile_rows, zchunk;

@@ -23,8 +23,13 @@ inline int hydro_tile_rows(int npy) {
     return w9 < w8 ? TILE_Y : 8;
 }
 // MHD tiles: 8 waves per workgroup (two per SIMD), so that the larger MHD
-// working set gets 256 VGPRs per lane instead of 168
-constexpr int TILE_Y_MAG = 5;
+// working set gets 256 VGPRs per lane instead of 168: 12 rows of 32 columns
+// (six flux waves of two rows each), or 5 rows of 64 columns (five flux
+// waves, the first tiling; 4% slower at 252x252x512, kept as an override)
+constexpr int TILE_Y_MAG = 12, TILE_W_MAG = 32;
+constexpr int TILE_Y_MAG64 = 5;
+// Columns of a stage tile of ty rows.
+inline int tile_width(bool lmag, int ty) { return lmag && ty == TILE_Y_MAG ? TILE_W_MAG : 64; }
 constexpr int DT_BLOCKS = 1024;
 
 // Set the calling thread's error message and return code.
@@ -74,7 +79,7 @@ struct r3d_hip_ctx {
     bool metrics_set = false;
     bool next_valid = false;  // scal->dt_next holds the dt of the current state
     std::vector<char> uploaded;
-    int ty = r3d::TILE_Y, ntx = 0, nty = 0, nkc = 0, kc = 0, nblocks = 0, grid = 0, nbb = 0;
+    int ty = r3d::TILE_Y, tw = 64, ntx = 0, nty = 0, nkc = 0, kc = 0, nblocks = 0, grid = 0, nbb = 0;
     unsigned long long bytes = 0;
     bool timing = false;
     long launches[4] = {0, 0, 0, 0};

@@ -342,7 +342,7 @@ void launch_stages_ty(r3d_ens_ctx* e, int stage, int cur) {
 int enqueue_step(r3d_ens_ctx* e, int cur) {
     const int n = (int)e->mem.size();
     for (int s = 0; s < 3; ++s) {
-        if (e->lmag) launch_stages_ty<TILE_Y_MAG, true>(e, s, cur);
+        if (e->lmag) launch_stages_ty<TILE_Y_MAG, true, TILE_W_MAG>(e, s, cur);
         else if (e->tile.tw == 32 && e->tile.ty == 8) launch_stages_ty<8, false, 32>(e, s, cur);
         else if (e->tile.tw == 32) launch_stages_ty<16, false, 32>(e, s, cur);
         else if (e->tile.tw == 16 && e->tile.ty == 16) launch_stages_ty<16, false, 16>(e, s, cur);
@@ -392,7 +392,7 @@ int launch_persistent_ty(r3d_ens_ctx* e, int nsteps) {
 }
 
 int launch_persistent(r3d_ens_ctx* e, int nsteps) {
-    if (e->lmag) return launch_persistent_ty<TILE_Y_MAG, true>(e, nsteps);
+    if (e->lmag) return launch_persistent_ty<TILE_Y_MAG, true, TILE_W_MAG>(e, nsteps);
     if (e->tile.tw == 32 && e->tile.ty == 8) return launch_persistent_ty<8, false, 32>(e, nsteps);
     if (e->tile.tw == 32) return launch_persistent_ty<16, false, 32>(e, nsteps);
     if (e->tile.tw == 16 && e->tile.ty == 16) return launch_persistent_ty<16, false, 16>(e, nsteps);
@@ -472,7 +472,7 @@ int r3d_ens_create(int nmembers, const r3d_hip_cfg* cfgs, r3d_ens_ctx** out) {
     auto* e = new r3d_ens_ctx();
     e->device = cfgs[0].device;
     e->lmag = cfgs[0].lmag;
-    e->tile = {e->lmag ? TILE_Y_MAG : TILE_Y, 0, 64};
+    e->tile = {e->lmag ? TILE_Y_MAG : TILE_Y, 0, e->lmag ? TILE_W_MAG : 64};
     auto fail = [&](int code) {
         r3d_ens_destroy(e);
         return code;
@@ -576,7 +576,7 @@ int r3d_ens_get_scalars(r3d_ens_ctx* e, int m, r3d_hip_scalars* s) {
 int r3d_ens_set_tile_shape(r3d_ens_ctx* e, int ty, int kc, int tw) {
     if (!e) return set_err(R3D_EINVAL, "null ensemble");
     bool ok;
-    if (e->lmag) ok = tw == 64 && ty == TILE_Y_MAG;
+    if (e->lmag) ok = tw == TILE_W_MAG && ty == TILE_Y_MAG;
     else if (tw == 64) ok = ty == 4 || ty == 8 || ty == 9;
     else if (tw == 32) ok = ty == 8 || ty == 16;
     else ok = tw == 16 && (ty == 16 || ty == 28);
@@ -590,7 +590,7 @@ int r3d_ens_set_tile_shape(r3d_ens_ctx* e, int ty, int kc, int tw) {
 }
 
 int r3d_ens_set_tiles(r3d_ens_ctx* e, int ty, int kc) {
-    return r3d_ens_set_tile_shape(e, ty, kc, 64);
+    return r3d_ens_set_tile_shape(e, ty, kc, e && e->lmag ? TILE_W_MAG : 64);
 }
 
 int r3d_ens_get_tile_shape(r3d_ens_ctx* e, int* ty, int* kc, int* tw, int* ntiles) {
@@ -710,7 +710,7 @@ int r3d_ens_autotune(r3d_ens_ctx* e, int trial_steps, double* best_ms) {
     };
     std::vector<Tiling> cand;
     if (e->lmag) {
-        for (int kc : {8, 16, 32, 64}) cand.push_back({TILE_Y_MAG, kc, 64});
+        for (int kc : {8, 16, 32, 64}) cand.push_back({TILE_Y_MAG, kc, TILE_W_MAG});
     } else {
         for (int ty : {4, 8, 9})
             for (int kc : {8, 16, 32, 64}) cand.push_back({ty, kc, 64});

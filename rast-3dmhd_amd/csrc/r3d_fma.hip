@@ -17,9 +17,10 @@ namespace r3d {
 // Launch one fused stage (same arguments and grid as the bit-exact kernel).
 hipError_t launch_stage_fma(const StageArgs& a, const Consts& C, const Metrics& M, int ty,
                             bool mag, unsigned grid, hipStream_t stream) {
-    const dim3 blk((ty + 3) * 64);
-#define R3D_FMA(TY, S, Z, MG) \
-    hipLaunchKernelGGL((r3d_fma::k_stage<TY, S, Z, MG>), dim3(grid), blk, 0, stream, a, C, M)
+#define R3D_FMA(TY, S, Z, MG)                                                                     \
+    hipLaunchKernelGGL((r3d_fma::k_stage<TY, S, Z, MG, false, false, MG ? TILE_W_MAG : 64>),     \
+                       dim3(grid), dim3(r3d_fma::TileShape<TY, MG ? TILE_W_MAG : 64>::NW * 64), 0, stream, a, \
+                       C, M)
 #define R3D_FMA_STAGES(TY, MG)                                        \
     if (a.stage == 0 && a.zmode == 0) R3D_FMA(TY, 0, 0, MG);          \
     else if (a.stage == 0) R3D_FMA(TY, 0, 1, MG);                     \

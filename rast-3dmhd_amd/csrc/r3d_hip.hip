@@ -504,11 +504,11 @@ StageArgs stage_args(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
 }
 
 // The five (stage, z mode) kernels of one instantiation family.
-template <int TY, bool MG, bool HT = false, bool DN = false>
+template <int TY, bool MG, bool HT = false, bool DN = false, int TW = 64>
 void launch_stage_kernel(r3d_hip_ctx* c, const StageArgs& a, int stage, int zmode) {
-    const dim3 blk((TY + 3) * 64), grid(c->grid);
+    const dim3 blk(TileShape<TY, TW>::NW * 64), grid(c->grid);
 #define R3D_STAGE(S, Z) \
-    hipLaunchKernelGGL((k_stage<TY, S, Z, MG, HT, DN>), grid, blk, 0, c->stream, a, c->C, c->M)
+    hipLaunchKernelGGL((k_stage<TY, S, Z, MG, HT, DN, TW>), grid, blk, 0, c->stream, a, c->C, c->M)
     if (stage == 0 && zmode == 0) R3D_STAGE(0, 0);
     else if (stage == 0) R3D_STAGE(0, 1);
     else if (stage == 1 && zmode == 0) R3D_STAGE(1, 0);
@@ -613,10 +613,12 @@ int launch_stage(r3d_hip_ctx* c, int stage, int zmode, int want_dt) {
             fma_err = launch_stage_fma(a, c->C, c->M, c->ty, c->cfg.lmag != 0, c->grid,
                                        c->stream);
         } else if (c->cfg.lmag) {
-            if (ht && dn) launch_stage_kernel<TILE_Y_MAG, true, true, true>(c, a, stage, zmode);
-            else if (ht) launch_stage_kernel<TILE_Y_MAG, true, true, false>(c, a, stage, zmode);
-            else if (dn) launch_stage_kernel<TILE_Y_MAG, true, false, true>(c, a, stage, zmode);
-            else launch_stage_kernel<TILE_Y_MAG, true>(c, a, stage, zmode);
+            constexpr int W = TILE_W_MAG;
+            if (ht && dn) launch_stage_kernel<TILE_Y_MAG, true, true, true, W>(c, a, stage, zmode);
+            else if (ht) launch_stage_kernel<TILE_Y_MAG, true, true, false, W>(c, a, stage, zmode);
+            else if (dn) launch_stage_kernel<TILE_Y_MAG, true, false, true, W>(c, a, stage, zmode);
+            else if (c->ty == TILE_Y_MAG64) launch_stage_kernel<TILE_Y_MAG64, true>(c, a, stage, zmode);
+            else launch_stage_kernel<TILE_Y_MAG, true, false, false, W>(c, a, stage, zmode);
         } else if (ht || dn) {  // these families are built with 8-row tiles
             if (ht && dn) launch_stage_kernel<8, false, true, true>(c, a, stage, zmode);
             else if (ht) launch_stage_kernel<8, false, true, false>(c, a, stage, zmode);
@@ -797,11 +799,14 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
         return set_err(R3D_EUNSUP, "flux relaxation with LREM (FTOT = 8.07*0.4*THETA) is not built");
     const bool feature = f.itcon == 2 || f.id != 0 || f.rlax > 0.0 || f.lrem;
     if (f.tile_rows != 0) {
-        const bool ok = f.lmag ? f.tile_rows == TILE_Y_MAG
+        const bool ok = f.lmag ? f.tile_rows == TILE_Y_MAG ||
+                                     (f.tile_rows == TILE_Y_MAG64 && !feature && f.variant == 0)
                                : feature ? f.tile_rows == 8 : (f.tile_rows == 8 || f.tile_rows == TILE_Y);
         if (!ok)
             return set_err(R3D_EINVAL, "tile_rows %d: built are %s", f.tile_rows,
-                           f.lmag ? "5 (LMAG)" : feature ? "8 (itcon 2, id, rlax)" : "8 and 9");
+                           f.lmag ? "12 (LMAG; 32 columns) and 5 (LMAG without itcon 2, id, rlax, "
+                                    "variant 0; 64 columns)"
+                           : feature ? "8 (itcon 2, id, rlax)" : "8 and 9");
     }
     if (!(f.dt_screen > -1.0))
         return set_err(R3D_EINVAL, "dt_screen %g: the margin must be > -1", f.dt_screen);
@@ -829,7 +834,8 @@ int r3d_hip_create(const r3d_hip_cfg* cfg, r3d_hip_ctx** out) {
     // stage tiling: 64 x ty columns, z chunks sized for >= ~8 blocks per CU
     c->ty = f.lmag ? TILE_Y_MAG : feature ? 8 : hydro_tile_rows(f.npy);
     if (f.tile_rows) c->ty = f.tile_rows;
-    c->ntx = (f.npx + 63) / 64;
+    c->tw = tile_width(f.lmag != 0, c->ty);
+    c->ntx = (f.npx + c->tw - 1) / c->tw;
     c->nty = (f.npy + c->ty - 1) / c->ty;
     c->kc = 8;
     for (int kc : {128, 64, 32, 16, 8}) {

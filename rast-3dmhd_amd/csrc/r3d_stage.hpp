@@ -829,13 +829,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
 }
 
 // One RK stage of one run: block -> tile with the XCD-aware order.
-template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false, bool DEN = false>
-__global__ __launch_bounds__((TY + 3) * 64) void k_stage(const StageArgs a, const Consts C,
-                                                         const Metrics M) {
-    __shared__ StageLds<TY, STAGE, MAG, HEAT, DEN> lds;
+template <int TY, int STAGE, int ZMODE, bool MAG, bool HEAT = false, bool DEN = false, int TW = 64>
+__global__ __launch_bounds__((TileShape<TY, TW>::NW * 64)) void k_stage(const StageArgs a,
+                                                                        const Consts C,
+                                                                        const Metrics M) {
+    __shared__ StageLds<TY, STAGE, MAG, HEAT, DEN, TW> lds;
     const int t = logical_block();
     if (t >= a.nblocks) return;
-    stage_tile<TY, STAGE, ZMODE, MAG, HEAT, DEN>(a, C, M, t, lds);
+    stage_tile<TY, STAGE, ZMODE, MAG, HEAT, DEN, TW>(a, C, M, t, lds);
 }
 
 }  // namespace R3D_KNS

@@ -43,9 +43,12 @@ def asm(tmp_path_factory):
     return out.read_text()
 
 
-def kernel_body(asm, ty, stage, zmode, mag=False, heat=False, den=False):
+def kernel_body(asm, ty, stage, zmode, mag=False, heat=False, den=False, tw=None):
+    """tw: tile columns (default: 32 for the 12-row MHD tile, else 64)."""
+    if tw is None:
+        tw = 32 if (mag and ty == 12) else 64
     name = (f"_ZN3r3d7k_stageILi{ty}ELi{stage}ELi{zmode}ELb{int(mag)}ELb{int(heat)}"
-            f"ELb{int(den)}EEEvNS_9StageArgsENS_6ConstsENS_7MetricsE")
+            f"ELb{int(den)}ELi{tw}EEEvNS_9StageArgsENS_6ConstsENS_7MetricsE")
     i = asm.index(name + ":")
     return asm[i:asm.index(".Lfunc_end", i)].split("\n")
 
@@ -89,11 +92,12 @@ def test_hydro_stage_kernel_memory_path(asm, ty, stage, zmode):
                          ids=["mhd", "hydro-heat", "mhd-heat", "hydro-den", "mhd-den",
                               "hydro-heat-den"])
 def test_other_stage_kernels_have_no_flat_access(asm, stage, zmode, mag, heat, den):
-    lines = kernel_body(asm, 5 if mag else 8, stage, zmode, mag=mag, heat=heat, den=den)
+    lines = kernel_body(asm, 12 if mag else 8, stage, zmode, mag=mag, heat=heat, den=den)
     assert not [l for l in lines if re.match(r"\s+flat_", l)], "FLAT memory instruction"
 
 
-@pytest.mark.parametrize("ty,mag", [(8, False), (9, False), (5, True)], ids=["hydro8", "hydro9", "mhd"])
+@pytest.mark.parametrize("ty,mag", [(8, False), (9, False), (12, True), (5, True)],
+                         ids=["hydro8", "hydro9", "mhd12x32", "mhd5x64"])
 @pytest.mark.parametrize("stage,zmode", FULL_STEP)
 def test_no_store_drain_after_the_plane_barrier(asm, ty, mag, stage, zmode):
     """Between a plane's barrier and its first RK-update store, no wait may

@@ -72,8 +72,11 @@ def randomize_field(orc, seed):
     orc.communicate()
 
 
-def upload_oracle_state(run, orc):
+def upload_oracle_state(run, orc, tile_rows=0):
+    """tile_rows: 0 = the library's MHD tile (12 rows x 32 columns), 5 = the
+    5-row, 64-column one."""
     cfg = run.cfg()
+    cfg.tile_rows = tile_rows
     sim = pkg.HipSim(cfg)
     sim.set_metrics(run.metrics())
     for r in range(orc.nranks):
@@ -96,15 +99,16 @@ def test_mhd_magnetic_layer_matches_oracle():
         compare(sim, orc, f"layer after +{n}")
 
 
+@pytest.mark.parametrize("tile_rows", [0, 5], ids=["tile12x32", "tile5x64"])
 @pytest.mark.parametrize("nstages", [1, 2, 3])
-def test_mhd_random_field_matches_oracle(nstages):
+def test_mhd_random_field_matches_oracle(nstages, tile_rows):
     """A fully three-dimensional B exercises every induction and Lorentz term."""
     args = ["--npx", "24", "--npy", "16", "--npz", "40", "--npey", "2", "--npez", "2"] + MAG + ROT
-    say(f"[gpu] mhd random field nstages={nstages}")
+    say(f"[gpu] mhd random field nstages={nstages} tile_rows={tile_rows}")
     run = pkg.Run(args)
     orc = oracle.Sim(args)
     randomize_field(orc, 7)
-    sim = upload_oracle_state(run, orc)
+    sim = upload_oracle_state(run, orc, tile_rows)
     compare(sim, orc, "after upload")
     sim.step(1, nstages)
     orc.step(1, nstages)
@@ -119,14 +123,15 @@ def test_mhd_random_field_matches_oracle(nstages):
     ["--npx", "70", "--npy", "18", "--npz", "30", "--npey", "3", "--npez", "3"],
     ["--npx", "9", "--npy", "8", "--npz", "9", "--npey", "2", "--npez", "3", "--izcon", "1"],
 ], ids=["single_z_rank", "ragged_x", "three_planes"])
-def test_mhd_edge_geometries_match_oracle(args):
+@pytest.mark.parametrize("tile_rows", [0, 5], ids=["tile12x32", "tile5x64"])
+def test_mhd_edge_geometries_match_oracle(args, tile_rows):
     """One z rank (no one-sided bottom stencils), partial tiles, 3-plane ranks."""
     args = args + MAG
-    say(f"[gpu] mhd edge case {' '.join(args)}")
+    say(f"[gpu] mhd edge case {' '.join(args)} tile_rows={tile_rows}")
     run = pkg.Run(args)
     orc = oracle.Sim(args)
     randomize_field(orc, 11)
-    sim = upload_oracle_state(run, orc)
+    sim = upload_oracle_state(run, orc, tile_rows)
     for n in (1, 2):
         sim.step(n)
         orc.step(n)

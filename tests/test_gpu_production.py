@@ -169,7 +169,7 @@ def test_rotating_mhd_workload_matches_oracle_252x252x512():
     sim = run.make_sim()
     run.close()
     ty, kc = sim.tiling()[:2]
-    assert (ty, kc) == (5, 32), (ty, kc)
+    assert (ty, kc) == (12, 32), (ty, kc)
     X = X5 + ["bx", "by", "bz"]
     Z = ["z" + n for n in X]
     done = 0
