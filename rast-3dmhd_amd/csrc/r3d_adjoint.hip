@@ -21,6 +21,8 @@
 //
 // The states X_n, X1, X2 of every step of one segment are recomputed from
 // the segment's checkpoint by one forward pass that keeps the stage states.
+// The last segment's are kept by the forward itself: its stages write into
+// the segment buffers, so the backward starts without a recompute.
 // Three adjoint buffers rotate through the stages, so that every flux
 // transpose reads its inputs from buffers it does not write.
 #include <hip/hip_runtime.h>
@@ -567,6 +569,22 @@ __global__ __launch_bounds__(AT) void k_bcon_adj(const Geo g, const State X, con
 
 // Objective per block (deterministic host sum) and, if LX is given, its
 // gradient: J = sum vol (ru^2+rv^2+rw^2)/(2 ro), vol = 1/(dxx dyy dzz).
+// X0's outer z planes (k = 0 and npz + 1) of every field into nbuf state
+// buffers: each buffer a stage writes into must hold them (the stage kernels
+// write interior cells only).
+__global__ __launch_bounds__(AT) void k_outer_planes(double* const* dst, int nbuf,
+                                                     const double* x0, long fe, int nf, long pp,
+                                                     int npz) {
+    const long per = 2 * pp;  // the two planes of one field
+    const long total = (long)nbuf * nf * per;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long)gridDim.x * blockDim.x) {
+        const long e = q % per, bf = q / per;
+        const long off = (bf % nf) * fe + (e < pp ? e : (long)(npz + 1) * pp + (e - pp));
+        dst[bf / nf][off] = x0[off];
+    }
+}
+
 __global__ __launch_bounds__(AT) void k_ke(const Geo g, const State X, const Metrics M,
                                            const State LX, int want_grad, double* part) {
 #pragma clang fp contract(off)
@@ -743,7 +761,10 @@ struct r3d_adj_ctx {
     std::vector<double> segdt;
     double* dt_pinned = nullptr;    // dt of the recomputed steps, read back once per segment
     double* scratch = nullptr;      // a state buffer for the unneeded last output of a segment
-    bool seg_outer = false;         // segment buffers hold X0's outer z planes
+    double** seg_bufs = nullptr;    // device list: the segment buffers and scratch
+    int nseg_bufs = 0;
+    bool seg_outer = false;         // segment buffers hold X0's outer z planes, xfinal all of X0
+    bool kept_last = false;         // segment buffers and segdt hold the last forward's last segment
     double *x0 = nullptr, *xfinal = nullptr;
     double* lam[3] = {nullptr, nullptr, nullptr};  // rotating adjoint buffers
     double* acc = nullptr;          // g_ore, g_tk, <F, lamF>, g_theta, dt's d/dRe, g_tb
@@ -800,20 +821,19 @@ int get_state(r3d_adj_ctx* a, double* dst) {
 // X0's values (RO's is rewritten by the boundary kernel every stage), so a
 // buffer that a stage writes into must hold them: copy X0's planes 0 and
 // npz+1 into every segment buffer, once per initial state.
+// The final state of the forward is the last stage's output buffer: it gets
+// the whole of X0 (ghost entries included), as the work context's buffers do
+// from put_state.
 int seg_outer_planes(r3d_adj_ctx* a) {
     if (a->seg_outer) return R3D_OK;
     r3d_hip_ctx* w = a->work;
-    const size_t pp = (size_t)w->g.pp;
-    std::vector<double*> bufs(a->segx);
-    bufs.insert(bufs.end(), a->seg1.begin(), a->seg1.end());
-    bufs.insert(bufs.end(), a->seg2.begin(), a->seg2.end());
-    bufs.push_back(a->scratch);
-    for (double* buf : bufs)
-            for (int f = 0; f < a->nf; ++f)
-                for (size_t k : {(size_t)0, (size_t)w->g.npz + 1})
-                    HIPCHK(hipMemcpyAsync(buf + f * a->fe + k * pp, a->x0 + f * a->fe + k * pp,
-                                          pp * sizeof(double), hipMemcpyDeviceToDevice,
-                                          w->stream));
+    const long total = (long)a->nseg_bufs * a->nf * 2 * w->g.pp;
+    hipLaunchKernelGGL(k_outer_planes, dim3((unsigned)std::min<long>(4096, (total + AT - 1) / AT)),
+                       dim3(AT), 0, w->stream, a->seg_bufs, a->nseg_bufs, a->x0, (long)a->fe,
+                       a->nf, (long)w->g.pp, w->g.npz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(a->xfinal, a->x0, a->fe * a->nf * sizeof(double),
+                          hipMemcpyDeviceToDevice, w->stream));
     a->seg_outer = true;
     return R3D_OK;
 }
@@ -941,6 +961,18 @@ int r3d_adj_create(const r3d_hip_cfg* cfg, int checkpoint_every, r3d_adj_ctx** o
     }
     a->segdt.assign(a->K, 0.0);
     if ((rc = alloc_state(a, &a->scratch))) return fail(rc);
+    {
+        std::vector<double*> bufs(a->segx);
+        bufs.insert(bufs.end(), a->seg1.begin(), a->seg1.end());
+        bufs.insert(bufs.end(), a->seg2.begin(), a->seg2.end());
+        bufs.push_back(a->scratch);
+        a->nseg_bufs = (int)bufs.size();
+        if (hipMalloc((void**)&a->seg_bufs, bufs.size() * sizeof(double*)) != hipSuccess)
+            return fail(set_err(R3D_ENOMEM, "adjoint: buffer list"));
+        if (hipMemcpy(a->seg_bufs, bufs.data(), bufs.size() * sizeof(double*),
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(R3D_EHIP, "adjoint: buffer list upload"));
+    }
     if (hipHostMalloc((void**)&a->dt_pinned, a->K * sizeof(double)) != hipSuccess)
         return fail(set_err(R3D_ENOMEM, "adjoint: pinned dt buffer"));
     // ghost entries of the initial state and the adjoint fields stay zero
@@ -959,6 +991,7 @@ int r3d_adj_destroy(r3d_adj_ctx* a) {
         if (p) (void)hipFree(p);
     if (a->dt_pinned) (void)hipHostFree(a->dt_pinned);
     if (a->bkey) (void)hipFree(a->bkey);
+    if (a->seg_bufs) (void)hipFree(a->seg_bufs);
     for (auto* v : {&a->ckpt, &a->segx, &a->seg1, &a->seg2})
         for (double* p : *v)
             if (p) (void)hipFree(p);
@@ -984,6 +1017,7 @@ int set_initial(r3d_adj_ctx* a, const double* const* fields, bool device) {
     }
     a->nsteps = 0;
     a->seg_outer = false;
+    a->kept_last = false;
     return R3D_OK;
 }
 }  // namespace
@@ -1008,16 +1042,46 @@ int r3d_adj_forward(r3d_adj_ctx* a, int nsteps, double* J) {
         if ((rc = alloc_state(a, &p))) return rc;
         a->ckpt.push_back(p);
     }
+    a->kept_last = false;
+    a->nsteps = 0;
+    if ((rc = seg_outer_planes(a))) return rc;
     w->cur = 0;
     if ((rc = put_state(a, a->x0))) return rc;
     r3d_hip_scalars s0{};
     if ((rc = r3d_hip_set_scalars(w, &s0))) return rc;
-    for (int n = 0; n < nsteps; ++n) {
+    const int n0 = (nck - 1) * a->K;  // first step of the last segment
+    for (int n = 0; n < n0; ++n) {
         if (n % a->K == 0 && (rc = get_state(a, a->ckpt[n / a->K]))) return rc;
         if ((rc = r3d_hip_step(w, 1, 3))) return rc;
     }
-    if ((rc = get_state(a, a->xfinal))) return rc;
+    // The last segment: its checkpoint, then its steps with the stage outputs
+    // redirected into the segment buffers (as the backward's recompute writes
+    // them) and the final state into xfinal.  The dt of step n0 is already
+    // reduced (next_valid is kept), so the steps are those of the plain path.
+    if ((rc = get_state(a, a->ckpt[nck - 1]))) return rc;
+    {
+        double* own[2][NXF];
+        std::memcpy(own, w->xs, sizeof own);
+        w->cur = 0;
+        for (int f = 0; f < a->nf; ++f) w->xs[0][f] = a->ckpt[nck - 1] + f * a->fe;
+        for (int n = n0; n < nsteps; ++n) {
+            double* outs[3] = {a->seg1[n - n0], a->seg2[n - n0],
+                               n + 1 < nsteps ? a->segx[n + 1 - n0] : a->xfinal};
+            rc = step_one(w, 3, nullptr, outs);
+            if (!rc && hipMemcpyAsync(a->dt_pinned + (n - n0), &w->scal->dt, sizeof(double),
+                                      hipMemcpyDeviceToHost, w->stream) != hipSuccess)
+                rc = set_err(R3D_EHIP, "dt readback failed");
+            if (rc) break;
+        }
+        std::memcpy(w->xs, own, sizeof own);
+        w->cur = 0;
+        w->next_valid = false;
+        if (rc) return rc;
+    }
     if ((rc = objective(a, a->xfinal, nullptr, &a->J))) return rc;
+    HIPCHK(hipStreamSynchronize(w->stream));
+    for (int n = n0; n < nsteps; ++n) a->segdt[n - n0] = a->dt_pinned[n - n0];
+    a->kept_last = true;
     a->nsteps = nsteps;
     if (J) *J = a->J;
     a->fwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
@@ -1049,29 +1113,33 @@ int backward(r3d_adj_ctx* a, double* const* grad, double* dJ_dre, double* dJ_dth
         // X_n, X1, X2 and dt of every step of this segment, from its
         // checkpoint.  The stages read and write the checkpoint and segment
         // buffers directly (the work context's state pointers are redirected,
-        // then restored), and dt comes back by one copy per segment.
-        double* own[2][NXF];
-        std::memcpy(own, w->xs, sizeof own);
-        w->cur = 0;
-        for (int f = 0; f < a->nf; ++f) w->xs[0][f] = a->ckpt[sgm] + f * a->fe;
-        w->next_valid = false;
-        for (int n = n0; n < n1; ++n) {
-            // the last step's X_{n+1} (a checkpoint or the final state) is
-            // stored already; it goes to a scratch buffer
-            double* outs[3] = {a->seg1[n - n0], a->seg2[n - n0],
-                               n + 1 < n1 ? a->segx[n + 1 - n0] : a->scratch};
-            rc = step_one(w, 3, nullptr, outs);
-            if (!rc && hipMemcpyAsync(a->dt_pinned + (n - n0), &w->scal->dt, sizeof(double),
-                                      hipMemcpyDeviceToHost, w->stream) != hipSuccess)
-                rc = set_err(R3D_EHIP, "dt readback failed");
-            if (rc) break;
+        // then restored), and dt comes back by one copy per segment.  The
+        // forward left the last segment's in place.
+        if (!(sgm == nseg - 1 && a->kept_last)) {
+            a->kept_last = false;  // the recompute overwrites the kept segment
+            double* own[2][NXF];
+            std::memcpy(own, w->xs, sizeof own);
+            w->cur = 0;
+            for (int f = 0; f < a->nf; ++f) w->xs[0][f] = a->ckpt[sgm] + f * a->fe;
+            w->next_valid = false;
+            for (int n = n0; n < n1; ++n) {
+                // the last step's X_{n+1} (a checkpoint or the final state) is
+                // stored already; it goes to a scratch buffer
+                double* outs[3] = {a->seg1[n - n0], a->seg2[n - n0],
+                                   n + 1 < n1 ? a->segx[n + 1 - n0] : a->scratch};
+                rc = step_one(w, 3, nullptr, outs);
+                if (!rc && hipMemcpyAsync(a->dt_pinned + (n - n0), &w->scal->dt, sizeof(double),
+                                          hipMemcpyDeviceToHost, w->stream) != hipSuccess)
+                    rc = set_err(R3D_EHIP, "dt readback failed");
+                if (rc) break;
+            }
+            std::memcpy(w->xs, own, sizeof own);
+            w->cur = 0;
+            w->next_valid = false;
+            if (rc) return rc;
+            HIPCHK(hipStreamSynchronize(w->stream));
+            for (int n = n0; n < n1; ++n) a->segdt[n - n0] = a->dt_pinned[n - n0];
         }
-        std::memcpy(w->xs, own, sizeof own);
-        w->cur = 0;
-        w->next_valid = false;
-        if (rc) return rc;
-        HIPCHK(hipStreamSynchronize(w->stream));
-        for (int n = n0; n < n1; ++n) a->segdt[n - n0] = a->dt_pinned[n - n0];
         for (int n = n1 - 1; n >= n0; --n) {
             const double* xn = n == n0 ? a->ckpt[sgm] : a->segx[n - n0];
             const double* x1b = a->seg1[n - n0];
@@ -1125,6 +1193,7 @@ int r3d_adj_set_fixed_dt(r3d_adj_ctx* a, double dt) {
     if (!(dt >= 0.0) || !std::isfinite(dt)) return set_err(R3D_EINVAL, "fixed dt %g", dt);
     a->work->C.dtfix = dt;
     a->nsteps = 0;  // a forward run under the new rule is needed before backward
+    a->kept_last = false;
     return R3D_OK;
 }
 
