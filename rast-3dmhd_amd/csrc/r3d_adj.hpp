@@ -744,6 +744,140 @@ __host__ __device__ inline void cell_flux_mag_extra_vjp(const Stencil& s, const 
     AB.ro2 -= w2 * ri2 * aw2;
 }
 
+// The wall extras of the LMAG transpose: RW and RO two planes in, BX and BY
+// two and three planes in (the one-sided stencils of a wall cell).
+struct MagWallAdj {
+    double rw2, ro2, bx2, by2, bx3, by3;
+};
+
+// cell_flux_mag_extra_vjp in point-major form, for kernels that add each
+// adjoint entry into an accumulator as soon as it is known: the cell's
+// coefficients (adjoints of the derivatives of B and of the velocities,
+// the resistive Laplacian weights) first, then each face point's seven
+// entries in closed form, so that no per-point adjoint array is held.
+// pt(q, ri, u, v, w) returns 1/RO and the velocities at face point q (a
+// kernel re-reads them where they are needed); out(q, f, value) receives the
+// entry of primary f (QRU..QRO, QBX..QBZ; the temperature's is zero) at face
+// point q, each (q, f) once.  The same entries as cell_flux_mag_extra_vjp,
+// which sums some of them in another order.
+template <class PT, class OUT>
+__host__ __device__ inline void cell_flux_mag_extra_vjp_points(const Stencil& s,
+                                                               const MagStencil& b,
+                                                               const CellMet& m, const Consts& C,
+                                                               const double lam[8], PT&& pt,
+                                                               OUT&& out, MagWallAdj& W) {
+    double u[NFACE], v[NFACE], w[NFACE], bx[NFACE], by[NFACE], bz[NFACE];
+    for (int q = 0; q < NFACE; ++q) {
+        u[q] = s.p[q].u;
+        v[q] = s.p[q].v;
+        w[q] = s.p[q].w;
+        bx[q] = b.p[q].bx;
+        by[q] = b.p[q].by;
+        bz[q] = b.p[q].bz;
+    }
+    const double hxd = C.hx * m.dxx, hyd = C.hy * m.dyy, hzd = C.hz * m.dzz;
+    const double lx1 = C.hx * m.d2xx, lx2 = C.h2x * m.dxx * m.dxx;
+    const double ly1 = C.hy * m.d2yy, ly2 = C.h2y * m.dyy * m.dyy;
+    const double lz1 = C.hz * m.d2zz, lz2 = C.h2z * m.dzz * m.dzz;
+    const double LU = lam[QRU], LV = lam[QRV], LW = lam[QRW], LT = lam[QTT];
+    const double L1 = lam[QBX], L2 = lam[QBY], L3 = lam[QBZ];
+    const double ob = C.obeta, oh = C.ohm, R = s.p[P_C].ri;
+    const double bx0 = bx[P_C], by0 = by[P_C], bz0 = bz[P_C];
+    const double u0 = u[P_C], v0 = v[P_C], w0 = w[P_C];
+    auto dx = [&](const double* q) { return (q[P_E] - q[P_W]) * hxd; };
+    auto dy = [&](const double* q) { return (q[P_N] - q[P_S]) * hyd; };
+    auto dz = [&](const double* q) { return (q[P_U] - q[P_D]) * hzd; };
+    const double bxy = dy(bx), byx = dx(by), bzx = dx(bz), bxz = dz(bx), byz = dz(by);
+    const double bzy = dy(bz), bxx = dx(bx), byy = dy(by), bzz = dz(bz);
+    const double ux = dx(u), vx = dx(v), wx = dx(w), uy = dy(u), vy = dy(v), wy = dy(w);
+    const double uz = dz(u), vz = dz(v);
+    const bool top = m.wall == 1, bot = m.wall == 2 && m.mag_bot;
+    const double ri2 = (top || bot) ? 1.0 / b.ro2 : 0.0;
+    const double w2 = s.rw2 * ri2;
+    double wzm = dz(w);
+    if (top) wzm = (4.0 * w[P_U] - w2) * hzd;
+    if (bot) wzm = (w2 - 4.0 * w[P_D]) * hzd;
+
+    // adjoints of the first derivatives of B
+    const double a_bxy = LU * ob * by0 - LV * ob * bx0 + LT * R * oh * (2.0 * bxy - 2.0 * byx) - L1 * v0;
+    const double a_byx = -LU * ob * by0 + LV * ob * bx0 + LT * R * oh * (2.0 * byx - 2.0 * bxy) - L2 * u0;
+    const double a_bzx = -LU * ob * bz0 + LW * ob * bx0 + LT * R * oh * (2.0 * bzx - 2.0 * bxz) - L3 * u0;
+    const double a_bxz = LU * ob * bz0 - LW * ob * bx0 + LT * R * oh * (2.0 * bxz - 2.0 * bzx) - L1 * w0;
+    const double a_byz = LV * ob * bz0 - LW * ob * by0 + LT * R * oh * (2.0 * byz - 2.0 * bzy) - L2 * w0;
+    const double a_bzy = -LV * ob * bz0 + LW * ob * by0 + LT * R * oh * (2.0 * bzy - 2.0 * byz) - L3 * v0;
+    const double a_bxx = -L1 * u0, a_byy = -L2 * v0, a_bzz = -L3 * w0;
+    // resistive Laplacian weights (orm times each component's adjoint)
+    const double Lb0 = L1 * C.orm, Lb1 = L2 * C.orm, Lb2 = L3 * C.orm;
+    // adjoints of the velocity derivatives
+    const double a_uy = L1 * by0, a_vy = -L1 * bx0 - L3 * bz0, a_uz = L1 * bz0;
+    const double a_wzm = -L1 * bx0 - L2 * by0, a_vx = L2 * bx0, a_ux = -L2 * by0 - L3 * bz0;
+    const double a_vz = L2 * bz0, a_wx = L3 * bx0, a_wy = L3 * by0;
+    // Ohmic heating's 1/RO at the cell
+    const double q1 = bxy * bxy + byx * byx + bzx * bzx - 2.0 * byx * bxy;
+    const double q2 = bxz * bxz - 2.0 * bxz * bzx;
+    const double q3 = byz * byz + bzy * bzy - 2.0 * byz * bzy;
+    const double aR = LT * oh * (q1 + q2 + q3);
+
+    // one face point: velocity adjoints to the momenta and the density with
+    // the point's own 1/RO and velocities, and the adjoint of B
+    auto point = [&](int q, double au, double av, double aw, double abx, double aby,
+                     double abz, double ro_extra) {
+        double ri, uq, vq, wq;
+        pt(q, ri, uq, vq, wq);
+        out(q, QRU, ri * au);
+        out(q, QRV, ri * av);
+        out(q, QRW, ri * aw);
+        out(q, QRO, -(ri * (uq * au + vq * av + wq * aw)) + ro_extra);
+        out(q, QBX, abx);
+        out(q, QBY, aby);
+        out(q, QBZ, abz);
+    };
+    // the cell itself: B of the induction and Lorentz terms, the velocities
+    // of the induction terms, every Laplacian's centre weight
+    {
+        const double cxy = -2.0 * lx2 - 2.0 * ly2;
+        const double czb = top ? -3.0 * lz1 + 2.0 * lz2 : (bot ? 3.0 * lz1 + 2.0 * lz2 : -2.0 * lz2);
+        const double abx = Lb0 * cxy + Lb0 * czb + (LV * ob * (byx - bxy) + LW * ob * (bzx - bxz) -
+                                                    L1 * vy - L1 * wzm + L2 * vx + L3 * wx);
+        const double aby = Lb1 * cxy + Lb1 * czb + (LU * ob * (bxy - byx) - LW * ob * (byz - bzy) +
+                                                    L1 * uy - L2 * ux - L2 * wzm + L3 * wy);
+        const double abz = Lb2 * cxy + Lb2 * (-2.0 * lz2) +
+                           (-LU * ob * bzx + LU * ob * bxz + LV * ob * (byz - bzy) + L1 * uz +
+                            L2 * vz - L3 * ux - L3 * vy);
+        point(P_C, -L1 * bxx - L2 * byx - L3 * bzx, -L1 * bxy - L2 * byy - L3 * bzy,
+              -L1 * bxz - L2 * byz - L3 * bzz, abx, aby, abz, -(R * R * aR));
+    }
+    const double le = lx1 + lx2, lw = -lx1 + lx2, ln = ly1 + ly2, ls = -ly1 + ly2;
+    point(P_E, a_ux * hxd, a_vx * hxd, a_wx * hxd, a_bxx * hxd + Lb0 * le, a_byx * hxd + Lb1 * le,
+          a_bzx * hxd + Lb2 * le, 0.0);
+    point(P_W, -(a_ux * hxd), -(a_vx * hxd), -(a_wx * hxd), -(a_bxx * hxd) + Lb0 * lw,
+          -(a_byx * hxd) + Lb1 * lw, -(a_bzx * hxd) + Lb2 * lw, 0.0);
+    point(P_N, a_uy * hyd, a_vy * hyd, a_wy * hyd, a_bxy * hyd + Lb0 * ln, a_byy * hyd + Lb1 * ln,
+          a_bzy * hyd + Lb2 * ln, 0.0);
+    point(P_S, -(a_uy * hyd), -(a_vy * hyd), -(a_wy * hyd), -(a_bxy * hyd) + Lb0 * ls,
+          -(a_byy * hyd) + Lb1 * ls, -(a_bzy * hyd) + Lb2 * ls, 0.0);
+    // vertical points: the one-sided wall stencils of W (for WZ) and of the
+    // Laplacians of BX and BY; BZ's Laplacian is centred everywhere
+    {
+        const double awu = top ? 4.0 * a_wzm * hzd : (bot ? 0.0 : a_wzm * hzd);
+        const double lu = top ? 4.0 * lz1 - 5.0 * lz2 : lz1 + lz2;
+        point(P_U, a_uz * hzd, a_vz * hzd, awu, a_bxz * hzd + (bot ? 0.0 : Lb0 * lu),
+              a_byz * hzd + (bot ? 0.0 : Lb1 * lu), a_bzz * hzd + Lb2 * (lz1 + lz2), 0.0);
+        const double awd = top ? 0.0 : (bot ? -(4.0 * a_wzm * hzd) : -(a_wzm * hzd));
+        const double ld = bot ? -4.0 * lz1 - 5.0 * lz2 : -lz1 + lz2;
+        point(P_D, -(a_uz * hzd), -(a_vz * hzd), awd, -(a_bxz * hzd) + (top ? 0.0 : Lb0 * ld),
+              -(a_byz * hzd) + (top ? 0.0 : Lb1 * ld), -(a_bzz * hzd) + Lb2 * (-lz1 + lz2), 0.0);
+    }
+    const double aw2 = top ? -(a_wzm * hzd) : (bot ? a_wzm * hzd : 0.0);
+    W.rw2 = ri2 * aw2;
+    W.ro2 = -(w2 * ri2 * aw2);
+    const double l2 = top ? -lz1 + 4.0 * lz2 : lz1 + 4.0 * lz2;
+    W.bx2 = (top || bot) ? Lb0 * l2 : 0.0;
+    W.by2 = (top || bot) ? Lb1 * l2 : 0.0;
+    W.bx3 = (top || bot) ? Lb0 * (-lz2) : 0.0;
+    W.by3 = (top || bot) ? Lb1 * (-lz2) : 0.0;
+}
+
 // Transpose of the whole MHD F (cell_flux_mag): the hydrodynamic part, then
 // the LMAG terms.
 __host__ __device__ inline void cell_flux_mag_vjp(const Stencil& s, const MagStencil& b,

@@ -332,23 +332,21 @@ __global__ __launch_bounds__(256) void k_vjp_tile(const Geo g, const State X, co
                 atomicAdd(out, v);
         }
     };
-    // add a cell's stencil entries (those vjp_mask allows) into the ring
-    auto emit = [&](int k, int np, const Stencil& A, const MagStencil* AB, bool lmag_terms) {
+    // add a cell's hydrodynamic stencil entries (those vjp_mask allows) into
+    // the ring
+    auto emit = [&](int k, const Stencil& A) {
 #pragma unroll
         for (int p = 0; p < NPTS; ++p) {
-            if (p >= np) break;
             int di, dj, dk;
             pt_offset(p, di, dj, dk);
             const int kk = k + dk;
             if (kk < 1 || kk > g.npz) continue;  // outer planes are not state
             double* rp = ring + slot(kk) * SLOT + (ly + 1 + dj) * VRW + (lx + 1 + di);
-            const unsigned msk = vjp_mask(p, MAG && !lmag_terms, lmag_terms);
-            const double vals[8] = {A.p[p].ru, A.p[p].rv, A.p[p].rw, A.p[p].ro, A.p[p].tt,
-                                    AB ? AB->p[p].bx : 0.0, AB ? AB->p[p].by : 0.0,
-                                    AB ? AB->p[p].bz : 0.0};
+            const unsigned msk = vjp_mask(p, false);
+            const double vals[5] = {A.p[p].ru, A.p[p].rv, A.p[p].rw, A.p[p].ro, A.p[p].tt};
 #pragma unroll
-            for (int f = 0; f < NF; ++f)
-                if (msk & (1u << f) && (f < 5 || AB)) atomicAdd(rp + f * VPL, vals[f]);
+            for (int f = 0; f < 5; ++f)
+                if (msk & (1u << f)) atomicAdd(rp + f * VPL, vals[f]);
         }
     };
 
@@ -427,7 +425,7 @@ __global__ __launch_bounds__(256) void k_vjp_tile(const Geo g, const State X, co
                 A.p[P_C].rw += own[QRW];
                 A.p[P_C].ro += own[QRO];
                 A.p[P_C].tt += own[QTT];
-                emit(k, NPTS, A, nullptr, false);
+                emit(k, A);
                 if (m.wall) atomicAdd(&LO.q[QRW][g.at(i, j, m.wall == 1 ? k + 2 : k - 2)], A.rw2);
             }
             // LMAG terms; the empty asm keeps the compiler from hoisting their
@@ -435,22 +433,23 @@ __global__ __launch_bounds__(256) void k_vjp_tile(const Geo g, const State X, co
             // would then be live together and spill)
             if constexpr (MAG) {
                 asm volatile("" ::: "memory");
-                Stencil s, A;
-                MagStencil b, AB;
-#pragma unroll
-                for (int p = 0; p < NFACE; ++p) {
+                Stencil s;
+                MagStencil b;
+                auto at_pt = [&](int p) {
                     int di, dj, dk;
                     pt_offset(p, di, dj, dk);
-                    const double* pl = sl[dk + 1] + (ly + 1 + dj) * VRW + (lx + 1 + di);
+                    return sl[dk + 1] + (ly + 1 + dj) * VRW + (lx + 1 + di);
+                };
+#pragma unroll
+                for (int p = 0; p < NFACE; ++p) {
+                    const double* pl = at_pt(p);
                     s.p[p].ri = pl[SS_RI * VPL];
                     s.p[p].u = pl[SS_U * VPL];
                     s.p[p].v = pl[SS_V * VPL];
                     s.p[p].w = pl[SS_W * VPL];
                     b.p[p] = {pl[SS_BX * VPL], pl[SS_BY * VPL], pl[SS_BZ * VPL]};
-                    A.p[p] = {0.0, 0.0, 0.0, 0.0, 0.0};
                 }
                 s.rw2 = rw2;
-                A.rw2 = 0.0;
                 b.ro2 = b.bx2 = b.by2 = b.bx3 = b.by3 = 0.0;
                 if (m.wall == 1 || m.mag_bot) {
                     const int dir = m.wall == 1 ? 1 : -1;
@@ -461,23 +460,45 @@ __global__ __launch_bounds__(256) void k_vjp_tile(const Geo g, const State X, co
                     b.bx3 = X.q[QBX][n3];
                     b.by3 = X.q[QBY][n3];
                 }
-                double E[8];
-                cell_flux_mag_extra(s, b, m, C, E);
-                cell_flux_mag_extra_vjp(s, b, m, C, lam, A, AB);
-                for (int f = 0; f < 8; ++f) dot += E[f] * lam[f];
-                AB.p[P_C].bx += own[QBX];
-                AB.p[P_C].by += own[QBY];
-                AB.p[P_C].bz += own[QBZ];
-                emit(k, NFACE, A, &AB, true);
+                {
+                    double E[8];
+                    cell_flux_mag_extra(s, b, m, C, E);
+                    for (int f = 0; f < 8; ++f) dot += E[f] * lam[f];
+                }
+                // point-major transpose: each face point's entries go into the
+                // ring as soon as they are known (the point's 1/RO and
+                // velocities re-read from the staged planes), so that no
+                // per-point adjoint array is held
+                MagWallAdj W;
+                cell_flux_mag_extra_vjp_points(
+                    s, b, m, C, lam,
+                    [&](int q, double& ri, double& u, double& v, double& w) {
+                        const double* pl = at_pt(q);
+                        ri = pl[SS_RI * VPL];
+                        u = pl[SS_U * VPL];
+                        v = pl[SS_V * VPL];
+                        w = pl[SS_W * VPL];
+                    },
+                    [&](int q, int f, double val) {
+                        int di, dj, dk;
+                        pt_offset(q, di, dj, dk);
+                        const int kk = k + dk;
+                        if (kk < 1 || kk > g.npz) return;  // outer planes are not state
+                        if (q == P_C && f >= QBX) val += own[f];
+                        atomicAdd(ring + slot(kk) * SLOT + (ly + 1 + dj) * VRW + (lx + 1 + di) +
+                                      f * VPL,
+                                  val);
+                    },
+                    W);
                 if (m.wall == 1 || m.mag_bot) {
                     const int dir = m.wall == 1 ? 1 : -1;
                     const long n2 = g.at(i, j, k + 2 * dir), n3 = g.at(i, j, k + 3 * dir);
-                    atomicAdd(&LO.q[QRW][n2], A.rw2);
-                    atomicAdd(&LO.q[QRO][n2], AB.ro2);
-                    atomicAdd(&LO.q[QBX][n2], AB.bx2);
-                    atomicAdd(&LO.q[QBY][n2], AB.by2);
-                    atomicAdd(&LO.q[QBX][n3], AB.bx3);
-                    atomicAdd(&LO.q[QBY][n3], AB.by3);
+                    atomicAdd(&LO.q[QRW][n2], W.rw2);
+                    atomicAdd(&LO.q[QRO][n2], W.ro2);
+                    atomicAdd(&LO.q[QBX][n2], W.bx2);
+                    atomicAdd(&LO.q[QBY][n2], W.by2);
+                    atomicAdd(&LO.q[QBX][n3], W.bx3);
+                    atomicAdd(&LO.q[QBY][n3], W.by3);
                 }
             }
         }

@@ -274,4 +274,66 @@ long adj_mask_violations(const int* dims, const double* const* X, const double* 
     return bad;
 }
 
+// cell_flux_mag_extra_vjp against its point-major form
+// (cell_flux_mag_extra_vjp_points) over every interior cell.  out[0]: the
+// largest difference of a face point's entry, out[1]: of a wall extra;
+// out[2], out[3]: the largest magnitude of each.  Returns the number of
+// (point, primary) entries the point-major form did not emit exactly once
+// (the temperature's: emitted at all).
+long adj_mag_points_diff(const int* dims, const double* const* X, const double* const* B,
+                         const double* const* met, const double* kc,
+                         const double* const* lamF, double* out) {
+    const Dims d{dims[0], dims[1], dims[2], dims[3], dims[4], dims[5]};
+    const Consts C = make_consts(kc);
+    long bad = 0;
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    for (int k = 1; k <= d.nz - 2; ++k)
+        for (int j = 1; j <= d.ny - 2; ++j)
+            for (int i = 1; i <= d.nx - 2; ++i) {
+                Stencil s, A;
+                MagStencil b, AB;
+                CellMet m;
+                gather(d, X, i, j, k, s, m, met);
+                gather_mag(d, X, B, i, j, k, m, b);
+                const long n = at(d, i, j, k);
+                double lam[8];
+                for (int q = 0; q < 8; ++q) lam[q] = lamF[q][n];
+                for (int q = 0; q < NPTS; ++q) A.p[q] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                A.rw2 = 0.0;
+                cell_flux_mag_extra_vjp(s, b, m, C, lam, A, AB);
+                double P[NFACE][8] = {};
+                int cnt[NFACE][8] = {};
+                MagWallAdj W;
+                cell_flux_mag_extra_vjp_points(
+                    s, b, m, C, lam,
+                    [&](int q, double& ri, double& u, double& v, double& w) {
+                        ri = s.p[q].ri;
+                        u = s.p[q].u;
+                        v = s.p[q].v;
+                        w = s.p[q].w;
+                    },
+                    [&](int q, int f, double val) {
+                        P[q][f] += val;
+                        ++cnt[q][f];
+                    },
+                    W);
+                for (int q = 0; q < NFACE; ++q) {
+                    const double ref[8] = {A.p[q].ru, A.p[q].rv, A.p[q].rw, A.p[q].ro, 0.0,
+                                           AB.p[q].bx, AB.p[q].by, AB.p[q].bz};
+                    for (int f = 0; f < 8; ++f) {
+                        if (cnt[q][f] != (f == QTT ? 0 : 1)) ++bad;
+                        out[0] = std::fmax(out[0], std::fabs(P[q][f] - ref[f]));
+                        out[2] = std::fmax(out[2], std::fabs(ref[f]));
+                    }
+                }
+                const double wg[6] = {W.rw2, W.ro2, W.bx2, W.by2, W.bx3, W.by3};
+                const double wr[6] = {A.rw2, AB.ro2, AB.bx2, AB.by2, AB.bx3, AB.by3};
+                for (int e = 0; e < 6; ++e) {
+                    out[1] = std::fmax(out[1], std::fabs(wg[e] - wr[e]));
+                    out[3] = std::fmax(out[3], std::fabs(wr[e]));
+                }
+            }
+    return bad;
+}
+
 }  // extern "C"

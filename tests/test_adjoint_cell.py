@@ -53,6 +53,8 @@ def lib(tmp_path_factory):
     L.adj_mask_violations.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, pp, dp, pp,
                                       ctypes.c_int]
     L.adj_mask_violations.restype = ctypes.c_long
+    L.adj_mag_points_diff.argtypes = [ctypes.POINTER(ctypes.c_int), pp, pp, pp, dp, pp, dp]
+    L.adj_mag_points_diff.restype = ctypes.c_long
     return L
 
 
@@ -273,3 +275,28 @@ def test_transpose_is_zero_outside_the_stored_mask(lib, sim, msim, mode):
                                       rk.k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                       _pp(lam), mode)
         assert bad == 0, (r, names, bad)
+
+
+def test_point_major_lmag_transpose_equals_term_major(lib, msim):
+    """cell_flux_mag_extra_vjp_points (the GPU transpose's form: each face
+    point's entries in closed form, emitted once) against the term-major
+    cell_flux_mag_extra_vjp, which the finite-difference checks above pin:
+    the same entries up to the order of a few sums, on every rank, walls with
+    their one-sided stencils included."""
+    rng = np.random.default_rng(123)
+    walls = 0.0
+    for r in range(msim.nranks):
+        rk = Rank(msim, r)
+        X = [msim.field(r, n).copy() for n in X_NAMES]
+        B = [msim.field(r, n).copy() for n in B_NAMES]
+        mask = _interior_mask(X[0].shape)
+        lam = [rng.uniform(-1, 1, X[0].shape) * mask for _ in range(8)]
+        out = np.zeros(4)
+        bad = lib.adj_mag_points_diff(rk.dims, _pp(X), _pp(B), _pp(rk.met),
+                                      rk.k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      _pp(lam), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        assert bad == 0, r
+        assert out[0] <= 1e-13 * out[2], (r, out)
+        assert out[1] <= 1e-13 * max(out[3], 1e-300), (r, out)
+        walls = max(walls, out[3])
+    assert walls > 0.0  # some rank holds a wall with one-sided LMAG stencils
