@@ -456,12 +456,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
             // One flux component at a time: each block reads only its own LDS
             // operands, accumulates its terms in the reference's order, applies
             // the RK update and stores, so that only finished values stay live
-            // (scheduling barriers keep the blocks apart in the hydrodynamic
-            // kernels).  With MAG the barriers are left out: at two waves per
-            // SIMD (the MHD tile's LDS and registers allow no more) the
-            // compiler then issues a block's LDS reads during the previous
-            // block, which hides their latency (7% faster); the hydrodynamic
-            // kernels, at three waves per SIMD, are slower without them.
+            // (one scheduling barrier, after FU, in the hydrodynamic kernels).
+            // Without barriers between the other blocks the compiler issues a
+            // block's LDS reads during the previous block, which hides their
+            // latency, and merges the derivatives that several blocks
+            // recompute from the same operands (same doubles).  Measured per
+            // stage launch: the MHD kernels (two waves per SIMD) 7% faster
+            // with no barrier at all; the hydrodynamic ones (three waves per
+            // SIMD) 3% faster with the one after FU than with one after every
+            // block, and 35-50% slower with none (their registers grow and
+            // the loads of the first blocks are scheduled late).
             // Derivatives that several components need (e.g. d(UU)/dx) are
             // recomputed from the same operands, which gives the same doubles.
             const bool bplane = (k == 1) || (k == g.npz);
@@ -542,7 +546,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 finish(QRO, fr, cc.ro);
             }
             
-            if constexpr (!MAG) __builtin_amdgcn_sched_barrier(0);
             {  // ---- FU (src/fluxes.cpp:149, 177-185, 289-297, 367, 449, 523) ----
                 const double uE = SH(SH_U, r, c + 1), uW = SH(SH_U, r, c - 1);
                 const double uN = SH(SH_U, r + 1, c), uS = SH(SH_U, r - 1, c);
@@ -596,7 +599,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 finish(QRV, fv, cc.rv);
             }
             
-            if constexpr (!MAG) __builtin_amdgcn_sched_barrier(0);
             {  // ---- FW (src/fluxes.cpp:129, 147, 213-221, 308-317, 496, 532) ----
                 const double wE = SH(SH_W, r, c + 1), wW = SH(SH_W, r, c - 1);
                 const double wN = SH(SH_W, r + 1, c), wS = SH(SH_W, r - 1, c);
@@ -623,7 +625,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 finish(QRW, fw, cc.rw);
             }
             
-            if constexpr (!MAG) __builtin_amdgcn_sched_barrier(0);
             {  // ---- FT (src/fluxes.cpp:235-272, 349-512) ----
                 const double rk = zm[3][kl], dk = zm[4][kl];
                 const double tt0 = cc.tt, ttU = cp.tt, ttD = cm.tt;
@@ -698,7 +699,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                 // the block, so these are scalar branches; the extra planes
                 // they need are read directly.
                 const bool wtop = k == 1, wbot = k == g.npz && C.mag_bot;
-                if constexpr (!MAG) __builtin_amdgcn_sched_barrier(0);
                 {  // ---- WW1, flux of BX (3dmhdsub.f:1080, 1130, 1230, 1280, 1370, 1410) ----
                     const double b2c = 2.0 * cc.bx;
                     double wzm = DZC(cp.w, cm.w);
@@ -726,7 +726,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                     w1 = w1 + C.orm * lbz;
                     finish(QBX, w1, cc.bx);
                 }
-                if constexpr (!MAG) __builtin_amdgcn_sched_barrier(0);
                 {  // ---- WW2, flux of BY (3dmhdsub.f:1030, 1140, 1240, 1290, 1380, 1420, 1490, 1530) ----
                     const double b2c = 2.0 * cc.by;
                     double wzm = DZC(cp.w, cm.w);
@@ -756,7 +755,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const Consts& C, 
                     w2 = w2 + C.orm * lbz;
                     finish(QBY, w2, cc.by);
                 }
-                if constexpr (!MAG) __builtin_amdgcn_sched_barrier(0);
                 {  // ---- WW3, flux of BZ (3dmhdsub.f:1040, 1090, 1250, 1310, 1500-1580) ----
                     const double b2c = 2.0 * cc.bz;
                     const double bE = SH(SH_BZ, r, c + 1), bW = SH(SH_BZ, r, c - 1);
