@@ -14,7 +14,11 @@
  *
  * Memory: the forward keeps a checkpoint every `checkpoint_every` steps; the
  * backward pass recomputes the states of one segment at a time from its
- * checkpoint, keeping the two intermediate stage states of every step.
+ * checkpoint, keeping the two intermediate stage states of every step.  The
+ * last segment's states are kept by the forward itself (its stages write
+ * into the segment buffers), so the first segment the backward transposes
+ * needs no recompute; the buffers hold 3 * checkpoint_every + 1 states
+ * either way.
  *
  * Hydrodynamic and MHD runs (with LMAG the state and the gradient have the
  * eight primaries R3D_RU..R3D_BZ); tc = 0.  Errors follow r3d_hip.h.
